@@ -1,0 +1,159 @@
+/* pcbgpu.h - C ABI of the MI355X grid-search back end of pcbroute.
+ *
+ * An exact shortest-path solver for one search window of a multi-layer clearance grid, used when a router's A*
+ * search has run past its pop threshold. It solves the same problem as pcbroute's CUDA back end
+ * (src/gpu.h:1-57, src/gpu.cu:1-622) on integer costs:
+ *
+ *   - a track move dm (0 +x, 1 -x, 2 +y, 3 -y, 4 (+,+), 5 (+,-), 6 (-,+), 7 (-,-)) from state p into state q on
+ *     layer l costs dirw[l][dm] * cf(q), where cf(q) is the resolved cost factor of q (src/gpu.cu:78-105);
+ *   - a via from any layer to layer l at cell c costs via_cost * vf(c) where a via may land on l
+ *     (src/gpu.cu:107-115, 331-350);
+ *   - distances are int32 and saturate at PCBGPU_INF.
+ *
+ * Plain pointers and sizes only: every pointer in a query is borrowed for the duration of the call and copied
+ * before it returns (the router's cost maps change under concurrent commits; src/router.cpp:190-195).
+ *
+ * Canonical result. Let D be the exact distance field of the solved rectangle, opt the least D over its target
+ * states and best the lowest state index among the targets with D == opt. The returned field holds D(s) where
+ * D(s) < opt, opt at best, and PCBGPU_INF everywhere else. It does not depend on the order in which the device
+ * relaxes the grid, so the HIP path and a CPU solve agree bit for bit.
+ */
+#ifndef PCBGPU_H
+#define PCBGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCBGPU_INF 0x3FFFFFFF        /* unreachable distance; a cost-map value >= it means blocked */
+#define PCBGPU_MAX_LAYERS 8
+#define PCBGPU_CF_MAX 0xFFFFF        /* resolved cost factors are clamped into [1, PCBGPU_CF_MAX] */
+#define PCBGPU_W_MAX 1023            /* largest move weight and via cost: keeps every edge cost below 2^30 */
+#define PCBGPU_FREE 0
+#define PCBGPU_BLOCK 0xFFFF
+
+/* return codes (src/gpu.h:54-57) */
+#define PCBGPU_ERR_FAIL (-1)         /* failure, window too big, bad query */
+#define PCBGPU_ERR_CHAIN (-2)        /* backtrace: the field does not explain a path (src/router.cpp:1654-1677) */
+#define PCBGPU_ERR_BUSY (-3)         /* every slot busy and wait == 0 */
+
+/* One entry of a state list: a window state (l * ww * wh + j * ww + i) or a window cell, and a value. */
+typedef struct pcbgpu_entry {
+    uint32_t state;
+    int32_t value;
+} pcbgpu_entry;
+
+/* The input of one solve. Mirrors GpuQuery (src/gpu.h:10-36) with integer costs:
+ *   float cst/vcst      -> int32 (>= PCBGPU_INF blocked, negative: some net's own copper, st says whose)
+ *   float step, dirf    -> int32 dirw[8][8] (the whole move weight: length, direction factor)
+ *   float mlf[256]      -> int32 mlbias[256], an additive cost-factor bias per prior value
+ *   std::vector members -> (pointer, count) pairs */
+typedef struct pcbgpu_query {
+    int32_t nx, ny, nl;                  /* the full grid; host arrays are [l][j * nx + i] */
+    int32_t wi0, wj0, ww, wh;            /* search window */
+    uint16_t me;                         /* the net as stored in the static grids */
+    int32_t via_cost;                    /* < 0: no vias */
+    int32_t dirw[PCBGPU_MAX_LAYERS][8];  /* per layer, weight of each move */
+    uint8_t lay[PCBGPU_MAX_LAYERS];      /* layers a track may use */
+    const int32_t* cst[PCBGPU_MAX_LAYERS];
+    const uint16_t* st[PCBGPU_MAX_LAYERS];
+    const int32_t* vcst;
+    const uint16_t* vst;
+    const uint64_t* tbit;                /* target bits over the window states */
+    const pcbgpu_entry* own_cf;          /* window state -> cost factor of the net's own cells */
+    size_t n_own_cf;
+    const pcbgpu_entry* own_vcf;         /* window cell -> via factor where the net's own vias are */
+    size_t n_own_vcf;
+    /* learned prior (null: none): pnl layers of ph rows of pw bytes, pixel (pi0, pj0) first; full-grid column / row
+     * -> prior pixel tables; router layer -> prior layer; cost bias per prior value; corridor threshold */
+    const uint8_t* prior;
+    int32_t pi0, pj0, pw, ph, pnl;
+    const int32_t* pcol;
+    const int32_t* prow;
+    int32_t play[PCBGPU_MAX_LAYERS];
+    int32_t mlbias[256];
+    uint8_t mthr;
+    uint8_t corridor;
+    const pcbgpu_entry* starts;          /* window state, initial cost g0 */
+    size_t n_starts;
+} pcbgpu_query;
+
+/* The solution. Mirrors GpuOut (src/gpu.h:39-50): the rectangle of the window that was solved, the cheapest target
+ * (a window state, -1: none) and the field over the rectangle ([l][j * rw + i], pinned host memory from a pool). */
+typedef struct pcbgpu_out {
+    int32_t rx0, ry0, rw, rh, nl, ww, wh;
+    int64_t best;
+    int32_t* d;
+    size_t cap;
+    int32_t sweeps;       /* passes of the last rectangle's solve */
+    int32_t tries;        /* rectangles solved */
+    double ms_upload, ms_solve, ms_total;
+    uint64_t tile_visits; /* tiles the sweep kernels processed (device work, all rectangles) */
+} pcbgpu_out;
+
+/* src/gpu.h:52 gpu_available: a device exists (the first call sets up the slots). */
+int pcbgpu_available(void);
+/* src/gpu.h:54 gpu_configure: concurrent solves (taken before the first use) and the most states a solve may take. */
+void pcbgpu_configure(int slots, size_t max_states);
+/* src/gpu.h:57 gpu_solve: sweeps >= 0, PCBGPU_ERR_FAIL, or PCBGPU_ERR_BUSY when every slot is busy and !wait. */
+int pcbgpu_solve(const pcbgpu_query* q, pcbgpu_out* out, int wait);
+/* GpuOut::~GpuOut (src/gpu.cu:405-407): gives the field buffer back to the pool. */
+void pcbgpu_out_release(pcbgpu_out* out);
+/* GpuOut::operator() (src/gpu.cu:409-416): cost of a window state, PCBGPU_INF outside the solved rectangle. */
+int32_t pcbgpu_out_at(const pcbgpu_out* out, int64_t window_state);
+
+/* The canonical backtrace (src/router.cpp:1602-1679), host code shared by every back end. From out->best, walks
+ * back taking at each state the first strict minimum over the track moves dm = 0..7 and then the vias from layers
+ * 0..nl-1; stops at a start whose field value is its g0. Fails (PCBGPU_ERR_CHAIN) when a state has no predecessor
+ * that explains its cost, when two vias of the chain are fewer than via_gapc steps apart, or when a via lies
+ * within sqrt(via_gap2) cells of one of old_vias (full-grid cells). Writes the chain target-first as (window
+ * state, code: 0-7 track move, 8 + l2 via from layer l2, 255 start) and returns its length. */
+int64_t pcbgpu_backtrace(const pcbgpu_query* q, const pcbgpu_out* out, int32_t via_gapc, double via_gap2,
+                         const uint32_t* old_vias, size_t n_old_vias, uint32_t* chain_states, uint8_t* chain_codes,
+                         size_t chain_cap);
+
+/* ---- device-resident grids (no counterpart in the reference: it copies the host maps for every solve,
+ * src/gpu.cu:432-439). The router's per-class cost maps live in HBM and are kept in step by span updates; a solve
+ * against a grid reads them there. */
+typedef struct pcbgpu_grid pcbgpu_grid;
+pcbgpu_grid* pcbgpu_grid_create(int32_t nx, int32_t ny, int32_t nl);
+void pcbgpu_grid_destroy(pcbgpu_grid* g);
+/* copy rows [j0, j0 + nrows) of layer l (l == nl: the via grid) from host arrays laid out like the query's */
+int pcbgpu_grid_upload(pcbgpu_grid* g, int32_t l, int32_t j0, int32_t nrows, const int32_t* cst, const uint16_t* st);
+/* a solve whose cst/st/vcst/vst come from the grid (the query's pointers for them are ignored) */
+int pcbgpu_solve_grid(const pcbgpu_grid* g, const pcbgpu_query* q, pcbgpu_out* out, int wait);
+/* one rectangle of the window, no widening (src/gpu.cu:422-546 solve_rect). flags: PCBGPU_NOFIELD leaves the
+ * canonical field on the device (out->d stays null; best and sweeps are still returned). */
+#define PCBGPU_NOFIELD 1
+int pcbgpu_solve_rect_grid(const pcbgpu_grid* g, const pcbgpu_query* q, int32_t rx0, int32_t ry0, int32_t rw,
+                           int32_t rh, pcbgpu_out* out, int flags);
+
+/* ---- tuning and diagnostics */
+/* tile rows per workgroup and local sweeps per tile visit; 0 keeps the current value. Returns 0 or -1. */
+int pcbgpu_set_tile(int32_t tile_rows, int32_t local_sweeps);
+void pcbgpu_get_tile(int32_t nl, int32_t* tile_rows, int32_t* local_sweeps);
+/* Kernel accounting, the counterpart of the reference's GPU_PROF timing (src/gpu.cu:491,504-516). While enabled,
+ * every sweep-pass launch is bracketed by HIP events on its stream and the states it processed are counted. */
+typedef struct pcbgpu_stats {
+    uint64_t solves;             /* rectangles solved */
+    uint64_t launches;           /* sweep-pass kernel launches (those queued past convergence included) */
+    uint64_t passes;             /* passes that ran before convergence was seen */
+    uint64_t tiles;              /* tile visits */
+    uint64_t states;             /* states carried through a full tile relaxation (the algorithmic unit) */
+    uint64_t rect_state_passes;  /* sum over rectangles of states x passes */
+    double kernel_ms;            /* summed duration of the sweep-pass launches (events) */
+    double solve_ms;             /* summed wall time of the rectangle solves */
+} pcbgpu_stats;
+void pcbgpu_stats_enable(int on);
+void pcbgpu_stats_read(pcbgpu_stats* s, int reset);
+
+/* last error text of the calling thread ("" if none) */
+const char* pcbgpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

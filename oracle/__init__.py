@@ -1,0 +1,131 @@
+"""ctypes binding of the CPU oracle (oracle/pcb_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the CPU-baseline leg of bench.py, as the
+checker of the HIP back end. The product package never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from pcbroute_amd.abi import INF, Out, Query
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+SRC = os.path.join(HERE, "pcb_oracle.c")
+
+
+def build(verbose=False):
+    tmp = LIB_PATH + ".tmp"
+    cmd = ["gcc", "-O2", "-std=c11", "-Wall", "-Wextra", "-fPIC", "-shared", "-o", tmp, SRC, "-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+class Oracle:
+    def __init__(self, path=LIB_PATH):
+        if not os.path.exists(path):
+            build()
+        lib = C.CDLL(path)
+        P = C.POINTER
+        lib.orc_resolve.argtypes = [P(Query), C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_int32), P(C.c_uint8),
+                                    P(C.c_uint8), P(C.c_int32)]
+        for f in (lib.orc_solve_rect, lib.orc_sweep_rect):
+            f.argtypes = [P(Query), C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_int32), P(C.c_int64), P(C.c_int32)]
+            f.restype = C.c_int
+        lib.orc_solve.argtypes = [P(Query), P(Out), C.c_size_t]
+        lib.orc_out_free.argtypes = [P(Out)]
+        lib.orc_backtrace.argtypes = [P(Query), P(Out), C.c_int32, C.c_double, P(C.c_uint32), C.c_size_t,
+                                      P(C.c_uint32), P(C.c_uint8), C.c_size_t]
+        lib.orc_backtrace.restype = C.c_int64
+        lib.orc_astar.argtypes = [P(Query), P(C.c_int64)]
+        lib.orc_astar.restype = C.c_int32
+        lib.orc_astar_batch.argtypes = [P(Query), C.c_int, C.c_int, P(C.c_int32), P(C.c_int64)]
+        self.lib = lib
+
+    def resolve(self, prob, rect=None):
+        rx0, ry0, rw, rh = rect or (0, 0, prob.window[2], prob.window[3])
+        N = prob.nl * rw * rh
+        cf = np.zeros(N, np.int32)
+        vok = np.zeros(N, np.uint8)
+        tg = np.zeros(N, np.uint8)
+        vcf = np.zeros(rw * rh, np.int32)
+        rc = self.lib.orc_resolve(C.byref(prob.query()), rx0, ry0, rw, rh, cf.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  vok.ctypes.data_as(C.POINTER(C.c_uint8)), tg.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                  vcf.ctypes.data_as(C.POINTER(C.c_int32)))
+        assert rc == 0
+        sh = (prob.nl, rh, rw)
+        return cf.reshape(sh), vok.reshape(sh), tg.reshape(sh), vcf.reshape(rh, rw)
+
+    def _rect(self, fn, prob, rect):
+        rx0, ry0, rw, rh = rect or (0, 0, prob.window[2], prob.window[3])
+        d = np.zeros(prob.nl * rw * rh, np.int32)
+        best = C.c_int64()
+        opt = C.c_int32()
+        rc = fn(C.byref(prob.query()), rx0, ry0, rw, rh, d.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(best),
+                C.byref(opt))
+        if rc < 0:
+            raise RuntimeError("oracle solve failed")
+        return d.reshape(prob.nl, rh, rw), int(best.value), int(opt.value), rc
+
+    def solve_rect(self, prob, rect=None):
+        """Dijkstra over the rectangle: (canonical field (nl, rh, rw), best rect state, opt)."""
+        d, best, opt, _ = self._rect(self.lib.orc_solve_rect, prob, rect)
+        return d, best, opt
+
+    def sweep_rect(self, prob, rect=None):
+        """Directional sweeps in the reference's order: (field, best, opt, sweeps)."""
+        return self._rect(self.lib.orc_sweep_rect, prob, rect)
+
+    def solve(self, prob, max_states=0):
+        """Whole solve with rectangle widening: dict(rect, best, tries, d)."""
+        out = Out()
+        rc = self.lib.orc_solve(C.byref(prob.query()), C.byref(out), max_states)
+        if rc < 0:
+            return None
+        n = out.nl * out.rw * out.rh
+        d = np.ctypeslib.as_array(out.d, shape=(n,)).copy().reshape(out.nl, out.rh, out.rw)
+        res = dict(rect=(out.rx0, out.ry0, out.rw, out.rh), best=int(out.best), tries=int(out.tries), d=d)
+        self.lib.orc_out_free(C.byref(out))
+        return res
+
+    def backtrace(self, prob, rect, best, d, via_gapc=0, via_gap2=0.0, old_vias=()):
+        out = Out()
+        out.rx0, out.ry0, out.rw, out.rh = rect
+        out.nl, out.ww, out.wh, out.best = prob.nl, prob.window[2], prob.window[3], best
+        d = np.ascontiguousarray(d, np.int32)
+        out.d = d.ctypes.data_as(C.POINTER(C.c_int32))
+        cap = max(1, d.size)
+        states = np.zeros(cap, np.uint32)
+        codes = np.zeros(cap, np.uint8)
+        ov = np.asarray(list(old_vias), np.uint32)
+        n = self.lib.orc_backtrace(C.byref(prob.query()), C.byref(out), int(via_gapc), float(via_gap2),
+                                   ov.ctypes.data_as(C.POINTER(C.c_uint32)), len(ov),
+                                   states.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                   codes.ctypes.data_as(C.POINTER(C.c_uint8)), cap)
+        if n < 0:
+            return int(n)
+        return [(int(states[k]), int(codes[k])) for k in range(n)]
+
+    def astar(self, prob):
+        pops = C.c_int64()
+        opt = self.lib.orc_astar(C.byref(prob.query()), C.byref(pops))
+        return int(opt), int(pops.value)
+
+    def astar_batch(self, probs, nthreads):
+        n = len(probs)
+        arr = (Query * n)()
+        for k, p in enumerate(probs):
+            arr[k] = p.query()
+        opt = np.zeros(n, np.int32)
+        pops = np.zeros(n, np.int64)
+        self.lib.orc_astar_batch(arr, n, int(nthreads), opt.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 pops.ctypes.data_as(C.POINTER(C.c_int64)))
+        return opt, pops
+
+
+__all__ = ["Oracle", "build", "INF"]

@@ -1,0 +1,129 @@
+"""HIP back end against the CPU oracle, bit for bit: canonical field, solved rectangle, best target, backtrace.
+
+Every case prints its seed and shape before it touches the GPU."""
+import numpy as np
+import pytest
+
+from pcbroute_amd import synth
+from pcbroute_amd.abi import INF
+
+from cases import random_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _same(res, ref):
+    assert res.rect == ref["rect"], (res.rect, ref["rect"])
+    assert res.best == ref["best"], (res.best, ref["best"])
+    bad = np.argwhere(res.d != ref["d"])
+    assert bad.size == 0, f"{len(bad)} states differ, first {bad[:5].tolist()}: gpu {res.d[tuple(bad[0])]} ref {ref['d'][tuple(bad[0])]}"
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_cases(backend, oracle, seed):
+    nl = 1 + seed % 4
+    p = random_case(seed, nx=96 + 8 * (seed % 5), ny=70 + 9 * (seed % 3), nl=nl)
+    _say("case", seed, p.window, p.nl)
+    ref = oracle.solve(p)
+    res = backend.solve(p)
+    if ref is None:
+        pytest.fail("oracle could not solve the case")
+    _same(res, ref)
+
+
+@pytest.mark.parametrize("rows,lsweeps", [(8, 1), (16, 2), (32, 8), (8, 64)])
+def test_tile_shapes_do_not_change_results(backend, oracle, rows, lsweeps):
+    p = random_case(100 + rows + lsweeps, nx=150, ny=131, nl=2)
+    _say("tiles", rows, lsweeps)
+    ref = oracle.solve(p)
+    backend.set_tile(rows, lsweeps)
+    try:
+        res = backend.solve(p)
+    finally:
+        backend.set_tile(-1, 8)
+    _same(res, ref)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_grid_resident_solves(backend, oracle, seed):
+    p = random_case(200 + seed, nx=120, ny=100, nl=1 + seed % 3)
+    _say("grid", seed)
+    g = backend.grid(p)
+    try:
+        res = backend.solve_grid(g, p)
+        ref = oracle.solve(p)
+        _same(res, ref)
+        rect = (3, 2, p.window[2] - 5, p.window[3] - 4)
+        rr = backend.solve_rect_grid(g, p, rect)
+        d, best, opt = oracle.solve_rect(p, rect)
+        assert rr.rect == rect
+        np.testing.assert_array_equal(rr.d, d)
+        if best >= 0:
+            rw, rh = rect[2], rect[3]
+            l, c = divmod(best, rw * rh)
+            assert rr.best == (l * p.window[3] + c // rw + rect[1]) * p.window[2] + c % rw + rect[0]
+        else:
+            assert rr.best == -1
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_backtrace_matches_oracle(backend, oracle, seed):
+    p = random_case(300 + seed, nx=100, ny=90, nl=2 + seed % 2)
+    _say("backtrace", seed)
+    res = backend.solve(p)
+    if res.best < 0:
+        pytest.skip("no target reachable in this case")
+    gap, gap2 = seed % 3, float(seed % 4)
+    mine = backend.backtrace(p, res, gap, gap2)
+    ref = oracle.backtrace(p, res.rect, res.best, res.d, gap, gap2)
+    assert mine == ref
+
+
+def test_unreachable_target(backend, oracle):
+    p = random_case(400, nx=64, ny=64, nl=2, features=False)
+    # wall the target in on every layer: its cell and ring blocked, no vias
+    cst = p.cst.copy()
+    st = p.st.copy()
+    cst[:, 20:31, 20:31] = INF
+    st[:, 20:31, 20:31] = 0xFFFF
+    from pcbroute_amd.abi import Problem
+    q = Problem(cst, st, p.vcst, p.vst, starts=[(p.state(0, 2, 2), 0)], targets=[p.state(1, 25, 25)], via_cost=-1)
+    _say("unreachable")
+    res = backend.solve(q)
+    ref = oracle.solve(q)
+    assert res.best == -1
+    _same(res, ref)
+
+
+def test_start_is_target(backend, oracle):
+    p = random_case(401, nx=80, ny=60, nl=2, features=False)
+    from pcbroute_amd.abi import Problem
+    s = p.state(1, 40, 30)
+    q = Problem(p.cst, p.st, p.vcst, p.vst, starts=[(s, 5)], targets=[s, p.state(0, 70, 50)])
+    _say("start is target")
+    res = backend.solve(q)
+    ref = oracle.solve(q)
+    _same(res, ref)
+    assert res.best == s
+
+
+@pytest.mark.parametrize("k", range(2))
+def test_full_size_kernel_board(backend, oracle, k):
+    """Two of the 64 searches of the 2000 x 1600 x 4 workload, whole grid, against the oracle's Dijkstra."""
+    board = synth.kernel_board(1)
+    p = synth.pair_problems(board, 7, 2)[k]
+    _say("full size", k)
+    g = backend.grid(p)
+    try:
+        res = backend.solve_rect_grid(g, p)
+    finally:
+        g.close()
+    d, best, opt = oracle.solve_rect(p)
+    assert res.best == best
+    np.testing.assert_array_equal(res.d, d)
