@@ -1,0 +1,63 @@
+"""The C ABI: libpcbgpu.so loads and exports every function include/pcbgpu.h declares, and the ctypes mirror of
+the structs has the C layout (offsets printed by a C program compiled here). No device calls."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from pcbroute_amd import LIB_PATH, load_library
+from pcbroute_amd.abi import Entry, Out, Query
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pcbgpu.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(pcbgpu_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_the_reference_entry_points():
+    names = declared()
+    for n in ("pcbgpu_available", "pcbgpu_configure", "pcbgpu_solve", "pcbgpu_out_release", "pcbgpu_out_at",
+              "pcbgpu_backtrace"):
+        assert n in names
+
+
+def test_library_exports_every_declared_function():
+    if not os.path.exists(LIB_PATH):
+        pytest.fail("libpcbgpu.so is not built (python -m pcbroute_amd.build)")
+    lib = load_library()
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (pcbgpu_[a-z_]+)", out))
+    missing = [n for n in declared() if n not in exported]
+    assert not missing, missing
+    for n in declared():
+        assert getattr(lib, n) is not None
+
+
+def test_struct_layout_matches_c():
+    fields = {
+        "pcbgpu_query": [f[0] for f in Query._fields_],
+        "pcbgpu_out": [f[0] for f in Out._fields_],
+        "pcbgpu_entry": [f[0] for f in Entry._fields_],
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void) {']
+    for s, fs in fields.items():
+        lines.append(f'printf("{s} size %zu\\n", sizeof({s}));')
+        for f in fs:
+            lines.append(f'printf("{s}.{f} %zu\\n", offsetof({s}, {f}));')
+    lines.append("return 0; }")
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "l.c"), os.path.join(d, "l")
+        open(src, "w").write("\n".join(lines))
+        subprocess.run(["gcc", "-o", exe, src], check=True)
+        got = dict(line.rsplit(" ", 1) for line in subprocess.run([exe], capture_output=True, text=True,
+                                                                   check=True).stdout.splitlines())
+    for s, cls in (("pcbgpu_query", Query), ("pcbgpu_out", Out), ("pcbgpu_entry", Entry)):
+        assert int(got[f"{s} size"]) == C.sizeof(cls), s
+        for f in fields[s]:
+            assert int(got[f"{s}.{f}"]) == getattr(cls, f).offset, (s, f)

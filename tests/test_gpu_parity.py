@@ -35,7 +35,7 @@ def test_random_cases(backend, oracle, seed):
     _same(res, ref)
 
 
-@pytest.mark.parametrize("rows,lsweeps", [(8, 1), (16, 2), (32, 8), (8, 64)])
+@pytest.mark.parametrize("rows,lsweeps", [(16, 1), (16, 2), (32, 8), (32, 64), (16, 64)])
 def test_tile_shapes_do_not_change_results(backend, oracle, rows, lsweeps):
     p = random_case(100 + rows + lsweeps, nx=150, ny=131, nl=2)
     _say("tiles", rows, lsweeps)
