@@ -47,8 +47,11 @@ def main():
     ap.add_argument("--pairs-seed", type=int, default=7)
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--local-sweeps", type=int, default=0)
+    ap.add_argument("--delta", type=int, default=-1, help="delta-stepping band over tiles (0: off)")
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=("single", "round"), default="single",
+                    help="single: the searches one after another; round: all of a step's searches in one round")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the search back end solves each search on a single device; "
@@ -61,6 +64,8 @@ def main():
         raise SystemExit("no HIP device")
     if a.tile_rows or a.local_sweeps:
         b.set_tile(a.tile_rows, a.local_sweeps)
+    if a.delta >= 0:
+        b.set_delta(a.delta)
     log("building the board")
     board = synth.kernel_board(a.board_seed)
     probs = synth.pair_problems(board, a.pairs_seed, a.pairs)
@@ -69,6 +74,8 @@ def main():
     log(f"board {nx}x{ny}x{nl}, {len(probs)} searches; tile rows {b.get_tile(nl)}")
 
     def step():
+        if a.mode == "round":
+            return b.round([(g, p) for p in probs], field=False, rects=[None] * len(probs))
         res = []
         for p in probs:
             res.append(b.solve_rect_grid(g, p, field=False))
@@ -109,7 +116,7 @@ def main():
                                f"{len(probs)} source/target pairs each solved alone over the whole grid",
                    "grid": [nx, ny, nl], "searches_per_step": len(probs), "obstacles": 0.2, "congested": 0.3,
                    "via_cost": synth.VIA, "move_weights": [synth.ORTH, synth.DIAG], "board_seed": a.board_seed,
-                   "pairs_seed": a.pairs_seed},
+                   "pairs_seed": a.pairs_seed, "mode": a.mode},
         "searches_per_s": round(n_search / wall, 3),
         "ms_per_search": round(wall / n_search * 1e3, 3),
         "passes_per_search": round(sum(sweeps) / len(sweeps), 2),

@@ -131,10 +131,32 @@ int pcbgpu_solve_grid(const pcbgpu_grid* g, const pcbgpu_query* q, pcbgpu_out* o
 int pcbgpu_solve_rect_grid(const pcbgpu_grid* g, const pcbgpu_query* q, int32_t rx0, int32_t ry0, int32_t rw,
                            int32_t rh, pcbgpu_out* out, int flags);
 
+/* ---- rounds: the speculative searches of one PathFinder round (src/router.cpp:2119-2242) in one sequence of
+ * launches. The searches are independent; each keeps its own field, bound and widening, and its result equals what
+ * pcbgpu_solve_grid returns for it alone. Submitted queries (and the grids) are borrowed until
+ * pcbgpu_round_launch returns. All searches of a round share one layer count. */
+typedef struct pcbgpu_round pcbgpu_round;
+pcbgpu_round* pcbgpu_round_begin(void);
+/* a ticket >= 0, or PCBGPU_ERR_FAIL */
+int pcbgpu_round_submit(pcbgpu_round* r, const pcbgpu_grid* g, const pcbgpu_query* q);
+/* the same for one given rectangle of the window, without widening (pcbgpu_solve_rect_grid's result) */
+int pcbgpu_round_submit_rect(pcbgpu_round* r, const pcbgpu_grid* g, const pcbgpu_query* q, int32_t rx0, int32_t ry0,
+                             int32_t rw, int32_t rh);
+/* PCBGPU_NOFIELD: the round's fields stay on the device (results keep best, rectangle and sweeps). 0 or -1. */
+int pcbgpu_round_flags(pcbgpu_round* r, int flags);
+/* solves every submitted search; 0 or PCBGPU_ERR_FAIL */
+int pcbgpu_round_launch(pcbgpu_round* r);
+/* the result of a ticket (its sweeps, or PCBGPU_ERR_FAIL); the field moves into out (release it as usual) */
+int pcbgpu_round_result(pcbgpu_round* r, int ticket, pcbgpu_out* out);
+void pcbgpu_round_end(pcbgpu_round* r);
+
 /* ---- tuning and diagnostics */
 /* tile rows per workgroup and local sweeps per tile visit; 0 keeps the current value. Returns 0 or -1. */
 int pcbgpu_set_tile(int32_t tile_rows, int32_t local_sweeps);
 void pcbgpu_get_tile(int32_t nl, int32_t* tile_rows, int32_t* local_sweeps);
+/* delta-stepping band over tiles: a pass visits the queued tiles whose key (the least cost change that queued them)
+ * lies within delta of the least key; 0 visits every queued tile. Results do not depend on it. 0 or -1. */
+int pcbgpu_set_delta(int32_t delta);
 /* Kernel accounting, the counterpart of the reference's GPU_PROF timing (src/gpu.cu:491,504-516). While enabled,
  * every sweep-pass launch is bracketed by HIP events on its stream and the states it processed are counted. */
 typedef struct pcbgpu_stats {

@@ -17,8 +17,9 @@ LIB_PATH = os.path.join(HERE, "libpcbgpu.so")
 EXPORTS = (
     "pcbgpu_available", "pcbgpu_configure", "pcbgpu_solve", "pcbgpu_out_release", "pcbgpu_out_at",
     "pcbgpu_backtrace", "pcbgpu_grid_create", "pcbgpu_grid_destroy", "pcbgpu_grid_upload", "pcbgpu_solve_grid",
-    "pcbgpu_solve_rect_grid", "pcbgpu_set_tile", "pcbgpu_get_tile", "pcbgpu_stats_enable", "pcbgpu_stats_read",
-    "pcbgpu_last_error",
+    "pcbgpu_solve_rect_grid", "pcbgpu_round_begin", "pcbgpu_round_submit", "pcbgpu_round_submit_rect", "pcbgpu_round_flags", "pcbgpu_round_launch",
+    "pcbgpu_round_result", "pcbgpu_round_end", "pcbgpu_set_tile", "pcbgpu_get_tile", "pcbgpu_set_delta", "pcbgpu_stats_enable",
+    "pcbgpu_stats_read", "pcbgpu_last_error",
 )
 
 
@@ -60,6 +61,20 @@ def load_library(path=LIB_PATH):
     lib.pcbgpu_set_tile.argtypes = [C.c_int32, C.c_int32]
     lib.pcbgpu_set_tile.restype = C.c_int
     lib.pcbgpu_get_tile.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    lib.pcbgpu_set_delta.argtypes = [C.c_int32]
+    lib.pcbgpu_set_delta.restype = C.c_int
+    lib.pcbgpu_round_begin.restype = C.c_void_p
+    lib.pcbgpu_round_submit.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Query)]
+    lib.pcbgpu_round_submit.restype = C.c_int
+    lib.pcbgpu_round_submit_rect.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Query), C.c_int32, C.c_int32,
+                                             C.c_int32, C.c_int32]
+    lib.pcbgpu_round_submit_rect.restype = C.c_int
+    lib.pcbgpu_round_flags.argtypes = [C.c_void_p, C.c_int]
+    lib.pcbgpu_round_launch.argtypes = [C.c_void_p]
+    lib.pcbgpu_round_launch.restype = C.c_int
+    lib.pcbgpu_round_result.argtypes = [C.c_void_p, C.c_int, C.POINTER(Out)]
+    lib.pcbgpu_round_result.restype = C.c_int
+    lib.pcbgpu_round_end.argtypes = [C.c_void_p]
     lib.pcbgpu_stats_enable.argtypes = [C.c_int]
     lib.pcbgpu_stats_read.argtypes = [C.POINTER(Stats), C.c_int]
     lib.pcbgpu_last_error.restype = C.c_char_p
@@ -145,6 +160,10 @@ class Backend:
         if self.lib.pcbgpu_set_tile(int(rows), int(local_sweeps)) != 0:
             raise BackendError("bad tile setting")
 
+    def set_delta(self, delta):
+        if self.lib.pcbgpu_set_delta(int(delta)) != 0:
+            raise BackendError("bad delta")
+
     def stats(self, enable=None, reset=False):
         """Kernel accounting: enable / disable it, and read (and optionally reset) the counters."""
         if enable is not None:
@@ -186,6 +205,36 @@ class Backend:
         rc = self.lib.pcbgpu_solve_rect_grid(grid.h, C.byref(prob.query()), rx0, ry0, rw, rh, C.byref(out),
                                              0 if field else NOFIELD)
         return self._finish(out, rc, field)
+
+    def round(self, pairs, field=True, rects=None):
+        """Solves [(grid, problem)] as one round (pcbgpu_round_*): a list of Results in submission order. rects:
+        one fixed rectangle per search (no widening), or None."""
+        r = self.lib.pcbgpu_round_begin()
+        if not r:
+            raise BackendError(self.error())
+        try:
+            if not field:
+                self.lib.pcbgpu_round_flags(r, NOFIELD)
+            tickets = []
+            for k, (g, p) in enumerate(pairs):
+                if rects is not None:
+                    rx0, ry0, rw, rh = rects[k] if rects[k] is not None else (0, 0, p.window[2], p.window[3])
+                    t = self.lib.pcbgpu_round_submit_rect(r, g.h, C.byref(p.query()), rx0, ry0, rw, rh)
+                else:
+                    t = self.lib.pcbgpu_round_submit(r, g.h, C.byref(p.query()))
+                if t < 0:
+                    raise BackendError(self.error())
+                tickets.append(t)
+            if self.lib.pcbgpu_round_launch(r) != 0:
+                raise BackendError(self.error())
+            out = []
+            for t in tickets:
+                o = Out()
+                rc = self.lib.pcbgpu_round_result(r, t, C.byref(o))
+                out.append(self._finish(o, rc, field))
+            return out
+        finally:
+            self.lib.pcbgpu_round_end(r)
 
     def backtrace(self, prob, res, via_gapc=0, via_gap2=0.0, old_vias=()):
         """Chain from res.best back to a start: list of (window state, code) or a negative error code."""

@@ -16,10 +16,12 @@
 namespace pcb {
 
 // The per-state word: bits 0-19 the cost factor of entering the state (0: it may not be entered), bit 20 a via may
-// land there, bit 21 it is a target.
+// land there, bit 21 it is a target, bit 30 it may not be entered (OR-ed into a candidate cost, it lifts the cost
+// above every bound, so a blocked cell needs no test of its own).
 constexpr uint32_t CF_MASK = 0xFFFFFu;
 constexpr uint32_t VOK_BIT = 1u << 20;
 constexpr uint32_t TG_BIT = 1u << 21;
+constexpr uint32_t BLK_BIT = 1u << 30;
 constexpr int32_t INF = PCBGPU_INF;
 
 struct ResolveParams {
@@ -59,7 +61,7 @@ PCB_HD uint32_t resolve_word(const ResolveParams& p, const int32_t* mlbias, int 
         else if (!blocked) r += mlbias[pv];
     }
     const bool vok = p.lay[l] ? (sok && (!corr || tg || pv >= p.mthr)) : bool(tg);
-    return (blocked ? 0u : clamp_cf(r)) | (vok ? VOK_BIT : 0u) | (tg ? TG_BIT : 0u);
+    return (blocked ? BLK_BIT : clamp_cf(r)) | (vok ? VOK_BIT : 0u) | (tg ? TG_BIT : 0u);
 }
 
 // prep_v (src/gpu.cu:107-115): the cost of a via at a cell, INF where none may be placed

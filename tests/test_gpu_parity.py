@@ -48,6 +48,18 @@ def test_tile_shapes_do_not_change_results(backend, oracle, rows, lsweeps):
     _same(res, ref)
 
 
+@pytest.mark.parametrize("delta", [1, 200, 3000])
+def test_delta_stepping_does_not_change_results(backend, oracle, delta):
+    probs = [random_case(700 + delta % 97 + k, nx=150, ny=120, nl=1 + k % 3) for k in range(4)]
+    _say("delta", delta)
+    backend.set_delta(delta)
+    try:
+        for p in probs:
+            _same(backend.solve(p), oracle.solve(p))
+    finally:
+        backend.set_delta(0)
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_grid_resident_solves(backend, oracle, seed):
     p = random_case(200 + seed, nx=120, ny=100, nl=1 + seed % 3)
@@ -127,3 +139,59 @@ def test_full_size_kernel_board(backend, oracle, k):
     d, best, opt = oracle.solve_rect(p)
     assert res.best == best
     np.testing.assert_array_equal(res.d, d)
+
+
+@pytest.mark.parametrize("nl", [2, 4])
+def test_round_equals_one_at_a_time(backend, oracle, nl):
+    """A round of searches with different windows and two grids (two net classes) gives each search exactly its
+    one-at-a-time result."""
+    probs = [random_case(600 + 10 * nl + k, nx=140, ny=110, nl=nl) for k in range(10)]
+    grids = [backend.grid(probs[0]), backend.grid(probs[1])]
+    try:
+        pairs = [(grids[k % 2], _on_grid(p, probs[k % 2])) for k, p in enumerate(probs)]
+        _say("round", nl, len(pairs))
+        got = backend.round(pairs)
+        for (g, p), r in zip(pairs, got):
+            one = backend.solve_grid(g, p)
+            assert r.rect == one.rect and r.best == one.best and r.sweeps == one.sweeps
+            np.testing.assert_array_equal(r.d, one.d)
+            ref = oracle.solve(p)
+            _same(r, ref)
+    finally:
+        for g in grids:
+            g.close()
+
+
+def _on_grid(p, other):
+    """p's search on other's maps (another net class's grid of the same board size)."""
+    from pcbroute_amd.abi import Problem
+    return Problem(other.cst, other.st, other.vcst, other.vst, starts=p.starts, targets=p.targets, window=p.window,
+                   me=p.me, via_cost=p.via_cost, dirw=p.dirw, lay=p.lay[:p.nl], own_cf=p.own_cf, own_vcf=p.own_vcf,
+                   prior=p.prior)
+
+
+def test_widening_solves_match(backend, oracle):
+    """Near pairs in a big window: the first rectangle is small and may widen; rectangle, best and field match."""
+    from pcbroute_amd.abi import Problem
+    board = synth.kernel_board(11, nx=600, ny=500, nl=2)
+    cst, st, vcst, vst = board
+    rng = np.random.default_rng(3)
+    probs = []
+    for k in range(4):
+        while True:
+            i, j = int(rng.integers(100, 500)), int(rng.integers(100, 400))
+            if st[0, j, i] == 0 and st[0, j + 20, i + 25] == 0:
+                break
+        probs.append(Problem(cst, st, vcst, vst, starts=[(j * 600 + i, 0)], targets=[(j + 20) * 600 + i + 25]))
+    g = backend.grid(probs[0])
+    try:
+        for p in probs:
+            _say("widening")
+            ref = oracle.solve(p)
+            _same(backend.solve(p), ref)
+            _same(backend.solve_grid(g, p), ref)
+        got = backend.round([(g, p) for p in probs])
+        for p, r in zip(probs, got):
+            _same(r, oracle.solve(p))
+    finally:
+        g.close()
