@@ -1,10 +1,15 @@
 """Single-search kernel throughput on one MI355X (the 2000 x 1600 x 4 workload).
 
 A step is the 64 seeded source / target searches of one synthetic board, each solved alone over the whole grid
-(12.8M states) on a device-resident grid; the timed region starts with the maps already in HBM. Prints one JSON
-line. --gpus N with N > 1 exits: this back end runs one search on one GPU.
+(12.8M states, its own field) on a device-resident grid; the timed region starts with the maps already in HBM.
+The searches of a step run as one round (one sequence of launches shared by all 64 searches, the batched path);
+the same searches one after another (one solve at a time) are timed beside it. Prints one JSON line.
 
-Usage: python bench.py [--steps K] [--warmup W] [--pairs P] [--no-cpu]
+value: grid states solved per second (64 x 12.8M states per step / wall time of the round mode).
+roofline: the sweep kernel k_pass, algorithmic bytes = 12 B per state carried through a tile visit (int32 cost word
+read, int32 field read and write) / summed kernel time from HIP events on the launch stream.
+
+--gpus N with N > 1 exits: the back end solves each search on a single device.
 """
 import argparse
 import json
@@ -16,7 +21,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-ALGO_BYTES = 12        # per state carried through one full relaxation: int32 cost read, int32 field read + write
+ALGO_BYTES = 12        # per state carried through a tile visit: int32 cost word read, int32 field read + write
+DEFAULT_DELTA = 0      # delta-stepping band over tiles: 0 = the library's automatic band (80 straight steps)
 
 
 def log(*a):
@@ -24,7 +30,8 @@ def log(*a):
 
 
 def cpu_baseline(board, pairs_seed, n_sample, threads):
-    """The oracle's A* (the CPU search the back end finishes) on a bounded sample of the same searches."""
+    """The oracle's A* (the CPU search the back end finishes) on a bounded sample of the same searches, and one
+    full-field Dijkstra (what the back end computes) on one thread."""
     from oracle import Oracle
     from pcbroute_amd import synth
     o = Oracle()
@@ -32,9 +39,34 @@ def cpu_baseline(board, pairs_seed, n_sample, threads):
     t0 = time.perf_counter()
     opt, pops = o.astar_batch(probs, threads)
     dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    o.solve_rect(probs[0])
+    dfull = time.perf_counter() - t1
     return dict(value=round(n_sample / dt, 3), unit="searches/s", cores=threads, kind="port",
-                sample=f"A* source-to-target on the first {n_sample} of the 64 searches, {threads} threads, "
-                       f"{dt:.1f} s, {int(pops.sum())} pops"), opt
+                sample=f"A* source-to-target (the CPU search the back end finishes) on the first {n_sample} of the "
+                       f"64 searches, {threads} threads, {dt:.1f} s, {int(pops.sum())} pops; one full-field "
+                       f"Dijkstra of the same grid on 1 thread: {dfull:.2f} s",
+                full_field_s=round(dfull, 3))
+
+
+def run(b, g, probs, mode, steps, warmup):
+    def step():
+        if mode == "round":
+            return b.round([(g, p) for p in probs], field=False, rects=[None] * len(probs))
+        return [b.solve_rect_grid(g, p, field=False) for p in probs]
+
+    for w in range(warmup):
+        step()
+        log(f"{mode}: warmup {w + 1}/{warmup} done")
+    b.stats(enable=True, reset=True)
+    t0 = time.perf_counter()
+    sweeps = []
+    for k in range(steps):
+        sweeps += [x.sweeps for x in step()]
+        log(f"{mode}: step {k + 1}/{steps} done")
+    wall = time.perf_counter() - t0
+    st = b.stats(enable=False, reset=True)
+    return wall, st, sweeps
 
 
 def main():
@@ -47,12 +79,23 @@ def main():
     ap.add_argument("--pairs-seed", type=int, default=7)
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--local-sweeps", type=int, default=0)
-    ap.add_argument("--delta", type=int, default=-1, help="delta-stepping band over tiles (0: off)")
+    ap.add_argument("--delta", type=int, default=DEFAULT_DELTA, help="delta-stepping band over tiles (0: automatic, -1: off)")
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=("single", "round"), default="single",
-                    help="single: the searches one after another; round: all of a step's searches in one round")
+    ap.add_argument("--mode", choices=("both", "single", "round"), default="both")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 4 times (calibrates the HBM byte counters) and exit")
     a = ap.parse_args()
+    if a.copy_probe:
+        import torch
+        n = a.copy_probe * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(4):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_mib": a.copy_probe, "copies": 4}), flush=True)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the search back end solves each search on a single device; "
               "run it with --gpus 1", file=sys.stderr)
@@ -64,44 +107,27 @@ def main():
         raise SystemExit("no HIP device")
     if a.tile_rows or a.local_sweeps:
         b.set_tile(a.tile_rows, a.local_sweeps)
-    if a.delta >= 0:
-        b.set_delta(a.delta)
+    b.set_delta(a.delta)
     log("building the board")
     board = synth.kernel_board(a.board_seed)
     probs = synth.pair_problems(board, a.pairs_seed, a.pairs)
     nl, ny, nx = board[0].shape
     g = b.grid(probs[0])
-    log(f"board {nx}x{ny}x{nl}, {len(probs)} searches; tile rows {b.get_tile(nl)}")
-
-    def step():
-        if a.mode == "round":
-            return b.round([(g, p) for p in probs], field=False, rects=[None] * len(probs))
-        res = []
-        for p in probs:
-            res.append(b.solve_rect_grid(g, p, field=False))
-        return res
-
-    for w in range(a.warmup):
-        step()
-        log(f"warmup {w + 1}/{a.warmup} done")
-    b.stats(enable=True, reset=True)
-    t0 = time.perf_counter()
-    sweeps = []
-    for k in range(a.steps):
-        r = step()
-        sweeps += [x.sweeps for x in r]
-        log(f"step {k + 1}/{a.steps} done")
-    wall = time.perf_counter() - t0
-    st = b.stats(enable=False, reset=True)
+    rows, lsw = b.get_tile(nl)
+    log(f"board {nx}x{ny}x{nl}, {len(probs)} searches; tile rows {rows}, local sweeps {lsw}, delta {a.delta}")
+    modes = ("round", "single") if a.mode == "both" else (a.mode,)
+    res = {m: run(b, g, probs, m, a.steps, a.warmup) for m in modes}
     g.close()
 
+    head = modes[0]
+    wall, st, sweeps = res[head]
     n_search = a.steps * len(probs)
-    gcells = st["states"] / wall / 1e9                       # whole job: state relaxations per second
+    cells = nx * ny * nl
     kern_s = st["kernel_ms"] / 1e3
     achieved = st["states"] * ALGO_BYTES / kern_s / 1e9 if kern_s > 0 else 0.0
     line = {
         "metric": "grid-search Gcells/s (% HBM roof)",
-        "value": round(gcells, 3),
+        "value": round(n_search * cells / wall / 1e9, 3),
         "unit": "Gcells/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -116,10 +142,13 @@ def main():
                                f"{len(probs)} source/target pairs each solved alone over the whole grid",
                    "grid": [nx, ny, nl], "searches_per_step": len(probs), "obstacles": 0.2, "congested": 0.3,
                    "via_cost": synth.VIA, "move_weights": [synth.ORTH, synth.DIAG], "board_seed": a.board_seed,
-                   "pairs_seed": a.pairs_seed, "mode": a.mode},
+                   "pairs_seed": a.pairs_seed, "mode": head, "tile_rows": rows, "local_sweeps": lsw,
+                   "delta": a.delta},
         "searches_per_s": round(n_search / wall, 3),
         "ms_per_search": round(wall / n_search * 1e3, 3),
         "passes_per_search": round(sum(sweeps) / len(sweeps), 2),
+        "tile_state_visits_per_search": round(st["states"] / n_search / cells, 2),
+        "processed_gstates_per_s": round(st["states"] / wall / 1e9, 3),
         "rect_state_passes_gcells_s": round(st["rect_state_passes"] / wall / 1e9, 3),
         "pct_hbm_roof": round(100 * achieved / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -128,9 +157,22 @@ def main():
                      "avg_launch_us": round(st["kernel_ms"] * 1e3 / max(1, st["launches"]), 3),
                      "algo_bytes_per_launch": round(st["states"] * ALGO_BYTES / max(1, st["launches"]))},
     }
+    # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this configuration
+    tpath = os.path.join(ROOT, "profiles", "k_pass_kernel_board_2000x1600x4_round.traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        want = {"tile_rows": rows, "local_sweeps": lsw, "delta": a.delta, "mode": head, "pairs": len(probs)}
+        if t.get("config") == want:
+            line["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
+            line["roofline"]["traffic_source"] = t["source"]
+    if "single" in res and head != "single":
+        w1, s1, sw1 = res["single"]
+        line["one_at_a_time"] = {"ms_per_search": round(w1 / n_search * 1e3, 3),
+                                 "gcells_s": round(n_search * cells / w1 / 1e9, 3),
+                                 "passes_per_search": round(sum(sw1) / len(sw1), 2)}
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        cb, opt = cpu_baseline(board, a.pairs_seed, min(a.cpu_sample, a.pairs), threads)
+        cb = cpu_baseline(board, a.pairs_seed, min(a.cpu_sample, a.pairs), threads)
         line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 

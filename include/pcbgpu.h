@@ -155,7 +155,8 @@ void pcbgpu_round_end(pcbgpu_round* r);
 int pcbgpu_set_tile(int32_t tile_rows, int32_t local_sweeps);
 void pcbgpu_get_tile(int32_t nl, int32_t* tile_rows, int32_t* local_sweeps);
 /* delta-stepping band over tiles: a pass visits the queued tiles whose key (the least cost change that queued them)
- * lies within delta of the least key; 0 visits every queued tile. Results do not depend on it. 0 or -1. */
+ * lies within delta of the least key; 0 picks 80 times the cheapest straight move weight (the default), -1 visits
+ * every queued tile. Results do not depend on it. 0 or -1. */
 int pcbgpu_set_delta(int32_t delta);
 /* Kernel accounting, the counterpart of the reference's GPU_PROF timing (src/gpu.cu:491,504-516). While enabled,
  * every sweep-pass launch is bracketed by HIP events on its stream and the states it processed are counted. */

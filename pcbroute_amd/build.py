@@ -13,6 +13,7 @@ ARCH = os.environ.get("PCBGPU_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-fPIC", "-Wall"]
 SOURCES = ["pcbgpu.hip", "backtrace.cpp", "gpu_shim.cpp"]
+CHECK = os.path.join(HERE, "shim_check")  # a C++ program that uses the gpu.h shim (built beside the library)
 
 
 def build_lib(verbose=False):
@@ -26,6 +27,14 @@ def build_lib(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed for libpcbgpu.so")
     os.replace(tmp, LIB)
+    cmd = ["g++", "-O2", "-std=c++17", "-o", CHECK, os.path.join(CSRC, "shim_check.cpp"), f"-L{HERE}", "-lpcbgpu",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed for shim_check")
     return LIB
 
 

@@ -48,7 +48,7 @@ def test_tile_shapes_do_not_change_results(backend, oracle, rows, lsweeps):
     _same(res, ref)
 
 
-@pytest.mark.parametrize("delta", [1, 200, 3000])
+@pytest.mark.parametrize("delta", [1, 200, 3000, -1])
 def test_delta_stepping_does_not_change_results(backend, oracle, delta):
     probs = [random_case(700 + delta % 97 + k, nx=150, ny=120, nl=1 + k % 3) for k in range(4)]
     _say("delta", delta)
@@ -195,3 +195,14 @@ def test_widening_solves_match(backend, oracle):
             _same(r, oracle.solve(p))
     finally:
         g.close()
+
+
+def test_cpp_gpu_h_shim():
+    """The C++ gpu.h shim (GpuQuery / GpuOut / gpu_solve) gives what the C ABI gives."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "pcbroute_amd", "shim_check")
+    assert os.path.exists(exe), "shim_check is not built (python -m pcbroute_amd.build)"
+    _say("cpp shim")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout + r.stderr
