@@ -7,7 +7,7 @@ from pcbroute_amd import synth
 from pcbroute_amd.abi import BLOCK, FREE, INF, MAX_LAYERS, Problem
 
 
-def random_case(seed, nx=96, ny=80, nl=3, features=True):
+def random_case(seed, nx=96, ny=80, nl=3, features=True, sym=False):
     rng = np.random.default_rng(seed)
     cst, st, vcst, vst = synth.kernel_board(seed, nx=nx, ny=ny, nl=nl)
     me = 7
@@ -31,9 +31,12 @@ def random_case(seed, nx=96, ny=80, nl=3, features=True):
         wi0, wj0, ww, wh = 0, 0, nx, ny
     dirw = np.zeros((MAX_LAYERS, 8), np.int32)
     for l in range(nl):
-        o = rng.integers(5, 40, 4)
-        dirw[l, :4] = o
-        dirw[l, 4:] = rng.integers(7, 60, 4)
+        if sym:  # the router's shape: +x/-x alike, +y/-y alike, the four diagonals alike
+            wx, wy, wd = (int(v) for v in rng.integers(5, 40, 3))
+            dirw[l] = [wx, wx, wy, wy, wd + 5, wd + 5, wd + 5, wd + 5]
+        else:
+            dirw[l, :4] = rng.integers(5, 40, 4)
+            dirw[l, 4:] = rng.integers(7, 60, 4)
     lay = np.ones(nl, np.uint8)
     if features and nl > 1 and rng.random() < 0.3:
         lay[int(rng.integers(nl))] = 0

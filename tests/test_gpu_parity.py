@@ -35,6 +35,14 @@ def test_random_cases(backend, oracle, seed):
     _same(res, ref)
 
 
+@pytest.mark.parametrize("seed", range(16))
+def test_random_cases_symmetric_weights(backend, oracle, seed):
+    """The kernel variant for sign-symmetric move weights (what the router produces)."""
+    p = random_case(900 + seed, nx=100 + 9 * (seed % 4), ny=74 + 7 * (seed % 3), nl=1 + seed % 5, sym=True)
+    _say("sym case", seed, p.window, p.nl)
+    _same(backend.solve(p), oracle.solve(p))
+
+
 @pytest.mark.parametrize("rows,lsweeps", [(16, 1), (16, 2), (32, 8), (32, 64), (16, 64)])
 def test_tile_shapes_do_not_change_results(backend, oracle, rows, lsweeps):
     p = random_case(100 + rows + lsweeps, nx=150, ny=131, nl=2)
