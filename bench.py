@@ -69,6 +69,63 @@ def run(b, g, probs, mode, steps, warmup):
     return wall, st, sweeps
 
 
+def round_workload(b, a):
+    """256 net searches of one speculative round (2-8 pins, windows log-uniform 64..2048 cells plus a margin) on a
+    2400x1800x6 board with two net classes: one round, then the same searches one at a time in submission order;
+    every result must agree bit for bit."""
+    import numpy as np
+    from pcbroute_amd import synth
+    log("building the board (two net classes)")
+    grids = synth.class_grids(a.board_seed)
+    searches = synth.net_searches(grids, a.pairs_seed, 256)
+    gs = [b.grid(synth_problem_of(g)) for g in grids]
+    pairs = [(gs[k], p) for k, p in searches]
+    states = sum(p.window[2] * p.window[3] * p.nl for _, p in searches)
+    log(f"{len(pairs)} searches, {states / 1e6:.0f}M window states")
+    b.round(pairs, field=False)  # warmup
+    b.stats(enable=True, reset=True)
+    t0 = time.perf_counter()
+    b.round(pairs, field=False)
+    t_round = time.perf_counter() - t0
+    st = b.stats(enable=False, reset=True)
+    log("round done")
+    t0 = time.perf_counter()
+    for pr in pairs:  # one at a time: a round of one search is the lone solve
+        b.round([pr], field=False)
+    t_one = time.perf_counter() - t0
+    log("one at a time done")
+    # exactness (untimed): fields, rectangles, best targets and sweeps of the round against lone solves
+    rr = b.round(pairs)
+    same = True
+    for (g, p), x in zip(pairs, rr):
+        y = b.solve_grid(g, p)
+        same = same and x.rect == y.rect and x.best == y.best and x.sweeps == y.sweeps and np.array_equal(x.d, y.d)
+    log("exactness check done")
+    for g in gs:
+        g.close()
+    kern_s = st["kernel_ms"] / 1e3
+    achieved = st["states"] * ALGO_BYTES / kern_s / 1e9 if kern_s > 0 else 0.0
+    print(json.dumps({
+        "metric": "batched speculative round searches/s", "value": round(len(pairs) / t_round, 2),
+        "unit": "searches/s", "n_gpus": 1, "higher_is_better": True, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "batched speculative round: 2400x1800x6, two net classes, 256 searches of 2-8 pins, "
+                               "boxes log-uniform 64..2048 cells", "board_seed": a.board_seed,
+                   "pairs_seed": a.pairs_seed, "window_states": states},
+        "round_s": round(t_round, 4), "one_at_a_time_s": round(t_one, 4), "speedup": round(t_one / t_round, 2),
+        "bit_exact_with_one_at_a_time": bool(same), "reached": int(sum(x.best >= 0 for x in rr)),
+        "tries_total": int(sum(x.tries for x in rr)),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "k_pass", "launches": st["launches"]},
+    }), flush=True)
+
+
+def synth_problem_of(grid):
+    """A Problem that only carries a class's maps (to create its device-resident grid)."""
+    from pcbroute_amd import Problem
+    cst, st, vcst, vst = grid
+    return Problem(cst, st, vcst, vst, starts=[], targets=[])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +140,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=("both", "single", "round"), default="both")
+    ap.add_argument("--workload", choices=("kernel", "round"), default="kernel",
+                    help="kernel: the 64 whole-grid searches (the metric); round: 256 net searches of a speculative "
+                         "round on a 2400x1800x6 board with two net classes, as one round and one at a time")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 4 times (calibrates the HBM byte counters) and exit")
     a = ap.parse_args()
@@ -108,6 +168,8 @@ def main():
     if a.tile_rows or a.local_sweeps:
         b.set_tile(a.tile_rows, a.local_sweeps)
     b.set_delta(a.delta)
+    if a.workload == "round":
+        return round_workload(b, a)
     log("building the board")
     board = synth.kernel_board(a.board_seed)
     probs = synth.pair_problems(board, a.pairs_seed, a.pairs)

@@ -81,3 +81,69 @@ def pair_problems(board, seed, n, via_cost=VIA, orth=ORTH, diag=DIAG):
         probs.append(Problem(cst, st, vcst, vst, starts=[(s, 0)], targets=[t], via_cost=via_cost,
                              dirw=default_dirw(nl, orth, diag)))
     return probs
+
+
+def class_grids(seed, nx=2400, ny=1800, nl=6, obstacles=0.05, clearances=(1, 2), congested=0.30, cmax=15):
+    """One board seen by several net classes: the same copper and keepouts (random blobs), grown by each class's
+    clearance in cells (its track half width plus clearance), so every class has its own static and cost maps.
+    Returns [(cst, st, vcst, vst)] per class."""
+    from scipy import ndimage
+    rng = np.random.default_rng(seed)
+    base = [blob_mask(rng, ny, nx, obstacles) for _ in range(nl)]
+    extra = [np.where(rng.random((ny, nx)) < congested, rng.integers(0, cmax + 1, (ny, nx)), 0) for _ in range(nl)]
+    out = []
+    for c in clearances:
+        k = np.ones((2 * c + 1, 2 * c + 1), bool)
+        cst = np.empty((nl, ny, nx), np.int32)
+        st = np.zeros((nl, ny, nx), np.uint16)
+        anyblk = np.zeros((ny, nx), bool)
+        esum = np.zeros((ny, nx), np.int64)
+        for l in range(nl):
+            blk = ndimage.binary_dilation(base[l], structure=k)
+            cst[l] = 1 + extra[l]
+            cst[l][blk] = INF
+            st[l][blk] = BLOCK
+            anyblk |= blk
+            esum += np.where(blk, 0, extra[l])
+        vcst = (1 + esum // nl).astype(np.int32)
+        vst = np.zeros((ny, nx), np.uint16)
+        vcst[anyblk] = INF
+        vst[anyblk] = BLOCK
+        out.append((cst, st, vcst, vst))
+    return out
+
+
+def net_searches(grids, seed, n=256, smin=64, smax=2048, margin=40, via_cost=VIA, orth=ORTH, diag=DIAG):
+    """n net searches of one speculative round: each net picks a class, a bounding box with sides log-uniform in
+    [smin, smax] cells and 2-8 pins at free states inside it; its search runs from the first pin to the others
+    inside the box grown by margin. Returns [(class index, Problem)]."""
+    rng = np.random.default_rng(seed)
+    nl, ny, nx = grids[0][0].shape
+    out = []
+    while len(out) < n:
+        k = int(rng.integers(len(grids)))
+        cst, st, vcst, vst = grids[k]
+        bw = int(min(nx, np.exp(rng.uniform(np.log(smin), np.log(smax)))))
+        bh = int(min(ny, np.exp(rng.uniform(np.log(smin), np.log(smax)))))
+        x0, y0 = int(rng.integers(0, nx - bw + 1)), int(rng.integers(0, ny - bh + 1))
+        npins = int(rng.integers(2, 9))
+        pins = []
+        for _ in range(50 * npins):
+            if len(pins) == npins:
+                break
+            l, i, j = int(rng.integers(nl)), x0 + int(rng.integers(bw)), y0 + int(rng.integers(bh))
+            if st[l, j, i] == FREE:
+                pins.append((l, i, j))
+        if len(pins) < 2:
+            continue
+        wi0, wj0 = max(0, x0 - margin), max(0, y0 - margin)
+        wi1, wj1 = min(nx, x0 + bw + margin), min(ny, y0 + bh + margin)
+        ww, wh = wi1 - wi0, wj1 - wj0
+
+        def ws(p):
+            l, i, j = p
+            return (l * wh + (j - wj0)) * ww + (i - wi0)
+
+        out.append((k, Problem(cst, st, vcst, vst, starts=[(ws(pins[0]), 0)], targets=[ws(p) for p in pins[1:]],
+                               window=(wi0, wj0, ww, wh), via_cost=via_cost, dirw=default_dirw(nl, orth, diag))))
+    return out
