@@ -1,0 +1,59 @@
+/* pcbprior.h - C ABI of the learned corridor / congestion prior: a small conv3x3 CNN trained and run on the MI355X
+ * with BF16 matrix-core (MFMA) kernels that accumulate in FP32.
+ *
+ * It replaces the ONNX Runtime call of pcbroute's learned prior (src/mlinfer.cpp:834-856: x (n,12,h,w) ->
+ * Session::Run -> (n,4,h,w) probabilities), with a network of its own: `depth` conv3x3 layers of `width` channels
+ * (ReLU between them), in_ch input channels per pixel, out_ch corridor logits per pixel (one per router layer).
+ * Pixels are the router grid downsampled 4x. Tensors cross the boundary as host float arrays, pixel-major (NHWC):
+ * x[((i * h + y) * w + x) * in_ch + c]. Spatial sizes: w a multiple of 64 (pad the crop).
+ *
+ * The prior enters a search only through pcbgpu_query's prior map (an integer cost bias per prior value, clamped to
+ * [0, 8], and a corridor threshold), and the plain search stays the fallback (src/router.cpp:1804).
+ */
+#ifndef PCBPRIOR_H
+#define PCBPRIOR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCBPRIOR_MAX_CH 32  /* channels a layer carries (inputs and outputs are padded to it) */
+
+typedef struct pcbprior pcbprior;
+
+/* A network of `depth` conv3x3 layers (in_ch -> width -> ... -> out_ch), weights drawn from `seed`. NULL on error. */
+pcbprior* pcbprior_create(int in_ch, int width, int depth, int out_ch, uint64_t seed);
+void pcbprior_destroy(pcbprior* p);
+
+/* Parameter count (FP32 master weights and biases, layer by layer: w[co][ky][kx][ci] then b[co]). */
+size_t pcbprior_num_params(const pcbprior* p);
+int pcbprior_get_params(const pcbprior* p, float* out);
+int pcbprior_set_params(pcbprior* p, const float* in);
+
+/* Forward pass: per-pixel corridor probabilities (sigmoid of the logits), out (n, h, w, out_ch). */
+int pcbprior_predict(pcbprior* p, const float* x, int n, int h, int w, float* prob);
+/* Logits instead of probabilities. */
+int pcbprior_logits(pcbprior* p, const float* x, int n, int h, int w, float* logits);
+
+/* Loss and gradient: mean binary cross-entropy of the logits against target (n, h, w, out_ch) in [0, 1] over the
+ * pixels where mask (n, h, w) is nonzero; grads gets d loss / d params in pcbprior_get_params's order. */
+int pcbprior_loss_grad(pcbprior* p, const float* x, const float* target, const float* mask, int n, int h, int w,
+                       float* loss, float* grads);
+
+/* One Adam step on a batch (the loss is the one above). */
+int pcbprior_train_step(pcbprior* p, const float* x, const float* target, const float* mask, int n, int h, int w,
+                        float lr, float* loss);
+
+/* One 16x16x16 BF16 matrix-core product, for the lane-layout test: a (16x16), b (16x16) row-major floats rounded
+ * to BF16, d (16x16) = a @ b in FP32. */
+int pcbprior_mfma_check(const float* a, const float* b, float* d);
+
+const char* pcbprior_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

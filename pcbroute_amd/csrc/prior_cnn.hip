@@ -1,0 +1,555 @@
+// prior_cnn.hip - the learned corridor prior: conv3x3 layers on BF16 matrix cores (MFMA 16x16x16, FP32
+// accumulation), forward, backward and Adam, behind include/pcbprior.h.
+//
+// Layout. Activations are pixel-major BF16 with 32 channel slots (NHWC, padded with zeros). A layer's weights are
+// kept as FP32 masters in the padded layout w[co][k] with k = (ky * 3 + kx) * 32 + ci, and as two BF16 copies: the
+// forward one in that layout and a flipped one wf[ci][k'] = w[co][(8 - d) * 32 + ci] with k' = d * 32 + co, which
+// makes the gradient with respect to a layer's input the same implicit GEMM as a forward pass.
+//
+// Implicit GEMM. A wavefront computes 16 consecutive pixels of a row times 32 output channels: two 16x16 tiles,
+// K = 9 positions x 32 input channels in 18 steps of 16. Operand fragments (MFMA 16x16x16): lane l holds A[l % 16][4
+// (l / 16) + j] and B[4 (l / 16) + j][l % 16] for j = 0..3, and gets D[4 (l / 16) + j][l % 16]; a lane's 4 A values
+// are 4 consecutive input channels of one pixel (8 bytes), its 4 B values 4 consecutive k of one output channel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/pcbprior.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+bool fail(const char* what, hipError_t e = hipSuccess) {
+    g_err = what;
+    if (e != hipSuccess) { g_err += ": "; g_err += hipGetErrorString(e); }
+    return false;
+}
+
+#define HIPOK(x)                                   \
+    do {                                           \
+        hipError_t e_ = (x);                       \
+        if (e_ != hipSuccess) return fail(#x, e_); \
+    } while (0)
+
+constexpr int C = PCBPRIOR_MAX_CH;  // channel slots of an activation
+constexpr int KK = 9 * C;           // the K of a layer's implicit GEMM
+constexpr int PX = 64;              // pixels of a row a workgroup takes (4 wavefronts x 16)
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// BF16 from FP32: round to nearest, ties to even (the IEEE rule); NaN stays a quiet NaN
+__host__ __device__ inline uint16_t f2bf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return uint16_t((u >> 16) | 0x40);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t h) {
+    const uint32_t u = uint32_t(h) << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ inline v4f mfma16(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+__device__ inline v4s load4(const uint16_t* p) {  // 4 consecutive BF16 values (8-byte aligned)
+    const uint2 u = *reinterpret_cast<const uint2*>(p);
+    v4s r;
+    r[0] = short(u.x & 0xFFFF); r[1] = short(u.x >> 16); r[2] = short(u.y & 0xFFFF); r[3] = short(u.y >> 16);
+    return r;
+}
+
+// ---------------------------------------------------------------- kernels
+
+__global__ void k_to_bf16(const float* x, uint16_t* out, size_t npix, int cin) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npix * C) return;
+    const size_t p = i / C;
+    const int c = int(i % C);
+    out[i] = c < cin ? f2bf(x[p * cin + c]) : uint16_t(0);
+}
+
+// y = conv3x3(x, wt) + bias: the implicit GEMM above. Outputs BF16 (32 slots, ReLU optional, zeroed where the mask
+// activation is zero: the derivative of the layer before's ReLU) or FP32 logits of the first cout channels.
+__global__ __launch_bounds__(256) void k_conv(const uint16_t* x, const uint16_t* wt, const float* bias, uint16_t* ybf,
+                                              float* yf, const uint16_t* mask, int relu, int cout, int H, int W) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = blockIdx.z, y = blockIdx.y, x0 = blockIdx.x * PX + wave * 16;
+    const int pr = lane & 15, kq = lane >> 4;  // A row (pixel) / B column (channel), k quarter
+    v4f acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+    const size_t img = size_t(n) * H * W;
+    for (int kk = 0; kk < KK / 16; ++kk) {
+        const int d = kk >> 1, half = kk & 1, dy = d / 3 - 1, dx = d % 3 - 1;
+        const int py = y + dy, px = x0 + pr + dx, ci = half * 16 + kq * 4;
+        v4s a = {0, 0, 0, 0};
+        if (py >= 0 && py < H && px >= 0 && px < W) a = load4(x + (img + size_t(py) * W + px) * C + ci);
+        const int k0 = kk * 16 + kq * 4;
+        const v4s b0 = load4(wt + size_t(pr) * KK + k0);
+        const v4s b1 = load4(wt + size_t(16 + pr) * KK + k0);
+        acc0 = mfma16(a, b0, acc0);
+        acc1 = mfma16(a, b1, acc1);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int co = 16 * t + pr;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int px = x0 + kq * 4 + j;
+            if (px >= W) continue;
+            const size_t p = img + size_t(y) * W + px;
+            float v = (t ? acc1[j] : acc0[j]) + bv;
+            if (relu) v = fmaxf(v, 0.f);
+            if (mask && (mask[p * C + co] & 0x7FFF) == 0) v = 0.f;  // the ReLU was off there
+            if (ybf) ybf[p * C + co] = f2bf(v);
+            if (yf && co < cout) yf[p * cout + co] = v;
+        }
+    }
+}
+
+// dW[co][k] += sum over the block's pixels of dy[p][co] * x_im2col[p][k]; db[co] += sum of dy[p][co]. A block takes
+// 64 columns of WG_ROWS rows, so that the FP32 atomics into dW are few next to the products. Per row it stages the
+// three input rows it needs (66 pixels with the halo) and the dy row in LDS; wavefront w then takes the k tiles w,
+// w + 4, ... against both co tiles. The reduction runs over pixels, so a lane's 4 operand values are 4 pixels of one
+// channel: LDS reads of one BF16 each.
+constexpr int WG_ROWS = 8;
+constexpr int XP = C + 2;  // LDS pitch of a pixel (BF16), padded against bank conflicts
+__global__ __launch_bounds__(256) void k_wgrad(const uint16_t* x, const uint16_t* dy, float* dw, float* db, int H, int W) {
+    __shared__ uint16_t xs[3][PX + 2][XP];
+    __shared__ uint16_t ds[PX][XP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = blockIdx.z, xb = blockIdx.x * PX;
+    const int pr = lane & 15, kq = lane >> 4;
+    const size_t img = size_t(n) * H * W;
+    constexpr int KT = KK / 16;  // 18 k tiles: wavefront w takes w, w + 4, ..., at most 5 of them
+    v4f acc[2][5];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 5; ++u) acc[t][u] = v4f{0, 0, 0, 0};
+    float bsum = 0.f;
+    const int y0 = blockIdx.y * WG_ROWS, y1 = min(H, y0 + WG_ROWS);
+    for (int y = y0; y < y1; ++y) {
+        // stage: 4 channels (8 bytes) per thread and step
+        for (int e = threadIdx.x; e < 3 * (PX + 2) * (C / 4); e += 256) {
+            const int r = e / ((PX + 2) * (C / 4)), rem = e % ((PX + 2) * (C / 4)), q = rem / (C / 4), c4 = rem % (C / 4);
+            const int py = y + r - 1, px = xb + q - 1;
+            uint2 v = make_uint2(0, 0);
+            if (py >= 0 && py < H && px >= 0 && px < W) v = *reinterpret_cast<const uint2*>(x + (img + size_t(py) * W + px) * C + c4 * 4);
+            uint16_t* d = &xs[r][q][c4 * 4];
+            d[0] = uint16_t(v.x & 0xFFFF); d[1] = uint16_t(v.x >> 16); d[2] = uint16_t(v.y & 0xFFFF); d[3] = uint16_t(v.y >> 16);
+        }
+        for (int e = threadIdx.x; e < PX * (C / 4); e += 256) {
+            const int q = e / (C / 4), c4 = e % (C / 4), px = xb + q;
+            uint2 v = make_uint2(0, 0);
+            if (px < W) v = *reinterpret_cast<const uint2*>(dy + (img + size_t(y) * W + px) * C + c4 * 4);
+            uint16_t* d = &ds[q][c4 * 4];
+            d[0] = uint16_t(v.x & 0xFFFF); d[1] = uint16_t(v.x >> 16); d[2] = uint16_t(v.y & 0xFFFF); d[3] = uint16_t(v.y >> 16);
+        }
+        __syncthreads();
+        if (threadIdx.x < C)
+            for (int q = 0; q < PX; ++q) bsum += bf2f(ds[q][threadIdx.x]);
+        for (int ps = 0; ps < PX / 16; ++ps) {
+            // A = dy^T (row co, column pixel), B = x_im2col (row pixel, column k)
+            v4s a0, a1;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                a0[j] = short(ds[ps * 16 + kq * 4 + j][pr]);
+                a1[j] = short(ds[ps * 16 + kq * 4 + j][16 + pr]);
+            }
+#pragma unroll
+            for (int u = 0; u < 5; ++u) {
+                const int kt = wave + 4 * u;
+                if (kt < KT) {
+                    const int kc = kt * 16 + pr, d = kc / C, ci = kc % C, r = d / 3, dxx = d % 3;  // halo: q = px + dxx
+                    v4s b;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) b[j] = short(xs[r][ps * 16 + kq * 4 + j + dxx][ci]);
+                    acc[0][u] = mfma16(a0, b, acc[0][u]);
+                    acc[1][u] = mfma16(a1, b, acc[1][u]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 5; ++u) {
+            const int kt = wave + 4 * u;
+            if (kt < KT)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) atomicAdd(&dw[size_t(16 * t + kq * 4 + j) * KK + kt * 16 + pr], acc[t][u][j]);
+        }
+    if (threadIdx.x < C) atomicAdd(&db[threadIdx.x], bsum);
+}
+
+// binary cross-entropy of the logits over the masked pixels; dz = (sigmoid(z) - t) * m / count in BF16 slots
+__global__ __launch_bounds__(256) void k_loss(const float* z, const float* t, const float* m, uint16_t* dz, float* loss,
+                                              size_t npix, int cout, float inv) {
+    __shared__ float part[4];
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float g = 0.f, l = 0.f;
+    if (i < npix * C) {
+        const size_t p = i / C;
+        const int c = int(i % C);
+        if (c < cout && m[p] != 0.f) {
+            const float zz = z[p * cout + c], tt = t[p * cout + c];
+            g = (1.f / (1.f + expf(-zz)) - tt) * inv;
+            l = (fmaxf(zz, 0.f) - zz * tt + log1pf(expf(-fabsf(zz)))) * inv;
+        }
+        dz[i] = f2bf(g);
+    }
+    for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, part[0] + part[1] + part[2] + part[3]);
+}
+
+__global__ void k_sigmoid(const float* z, float* p, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 1.f / (1.f + expf(-z[i]));
+}
+
+// Adam on the padded FP32 masters; refreshes the forward and flipped BF16 copies
+__global__ void k_adam(float* w, const float* g, float* mo, float* ve, size_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gg = g[i];
+    const float m = b1 * mo[i] + (1.f - b1) * gg, v = b2 * ve[i] + (1.f - b2) * gg * gg;
+    mo[i] = m;
+    ve[i] = v;
+    w[i] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+__global__ void k_weights_bf16(const float* w, uint16_t* wt, uint16_t* wf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // over C * KK
+    if (i >= C * KK) return;
+    const int co = i / KK, k = i % KK, d = k / C, ci = k % C;
+    wt[i] = f2bf(w[i]);
+    wf[size_t(ci) * KK + (8 - d) * C + co] = f2bf(w[i]);  // wf[ci][d' * 32 + co] = w[co][(8 - d') * 32 + ci]
+}
+
+__global__ void k_mfma_check(const uint16_t* a, const uint16_t* b, float* d) {
+    const int lane = threadIdx.x, pr = lane & 15, kq = lane >> 4;
+    v4s fa, fb;
+    for (int j = 0; j < 4; ++j) {
+        fa[j] = short(a[pr * 16 + kq * 4 + j]);  // A[pr][4 kq + j]
+        fb[j] = short(b[(kq * 4 + j) * 16 + pr]);  // B[4 kq + j][pr]
+    }
+    const v4f r = mfma16(fa, fb, v4f{0, 0, 0, 0});
+    for (int j = 0; j < 4; ++j) d[(kq * 4 + j) * 16 + pr] = r[j];  // D[4 kq + j][pr]
+}
+
+unsigned g1(size_t n) { return unsigned((n + 255) / 256); }
+
+template <class T>
+struct DB {  // a device array grown on demand
+    T* p = nullptr;
+    size_t cap = 0;
+    bool need(size_t n) {
+        if (n == 0) n = 1;
+        if (cap >= n) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) { p = nullptr; return fail("hipMalloc"); }
+        cap = n;
+        return true;
+    }
+    ~DB() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+struct pcbprior {
+    int cin = 0, width = 0, depth = 0, cout = 0;
+    std::vector<int> lin, lout;  // real channels of each layer
+    hipStream_t s = nullptr;
+    DB<float> w, g, mo, ve, bias, gb, mb, vb;  // per layer: C * KK weights, C biases (padded)
+    DB<uint16_t> wt, wf;
+    std::vector<DB<uint16_t>> act;  // depth activations: act[0] the input, act[L] the input of layer L
+    DB<uint16_t> dya, dyb;
+    DB<float> logits, tgt, msk, xin, loss;
+    int step = 0;
+    size_t npix = 0;
+};
+
+namespace {
+
+// every copy goes through the network's own stream (non-blocking: the null stream does not order it)
+bool h2d(pcbprior* p, void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, p->s) == hipSuccess || fail("hipMemcpyAsync H2D");
+}
+bool d2h(pcbprior* p, void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, p->s) == hipSuccess || fail("hipMemcpyAsync D2H");
+}
+bool sync(pcbprior* p) { return hipStreamSynchronize(p->s) == hipSuccess || fail("hipStreamSynchronize"); }
+
+bool ensure(pcbprior* p, int n, int h, int w) {
+    if (n < 1 || h < 1 || w < 1 || w % PX) return fail("n, h >= 1 and w a positive multiple of 64");
+    const size_t np = size_t(n) * h * w;
+    p->npix = np;
+    if (int(p->act.size()) != p->depth) p->act.resize(p->depth);
+    for (auto& a : p->act)
+        if (!a.need(np * C)) return false;
+    return p->dya.need(np * C) && p->dyb.need(np * C) && p->logits.need(np * p->cout) && p->tgt.need(np * p->cout) &&
+           p->msk.need(np) && p->xin.need(np * p->cin) && p->loss.need(1);
+}
+
+bool refresh_bf16(pcbprior* p) {
+    for (int L = 0; L < p->depth; ++L)
+        hipLaunchKernelGGL(k_weights_bf16, dim3(g1(C * KK)), dim3(256), 0, p->s, p->w.p + size_t(L) * C * KK,
+                           p->wt.p + size_t(L) * C * KK, p->wf.p + size_t(L) * C * KK);
+    HIPOK(hipGetLastError());
+    return true;
+}
+
+bool forward(pcbprior* p, const float* x, int n, int h, int w) {
+    if (!ensure(p, n, h, w)) return false;
+    const size_t np = p->npix;
+    HIPOK(hipMemcpyAsync(p->xin.p, x, np * p->cin * 4, hipMemcpyHostToDevice, p->s));
+    hipLaunchKernelGGL(k_to_bf16, dim3(g1(np * C)), dim3(256), 0, p->s, p->xin.p, p->act[0].p, np, p->cin);
+    const dim3 grid(w / PX, h, n);
+    for (int L = 0; L < p->depth; ++L) {
+        const bool last = L == p->depth - 1;
+        hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, p->s, p->act[L].p, p->wt.p + size_t(L) * C * KK,
+                           p->bias.p + L * C, last ? nullptr : p->act[L + 1].p, last ? p->logits.p : nullptr,
+                           static_cast<const uint16_t*>(nullptr), last ? 0 : 1, p->cout, h, w);
+    }
+    HIPOK(hipGetLastError());
+    return true;
+}
+
+bool backward(pcbprior* p, const float* target, const float* mask, int n, int h, int w, float* loss_out) {
+    const size_t np = p->npix;
+    HIPOK(hipMemcpyAsync(p->tgt.p, target, np * p->cout * 4, hipMemcpyHostToDevice, p->s));
+    HIPOK(hipMemcpyAsync(p->msk.p, mask, np * 4, hipMemcpyHostToDevice, p->s));
+    std::vector<float> mh(mask, mask + np);
+    double cnt = 0;
+    for (float v : mh) cnt += v != 0.f;
+    const float inv = cnt > 0 ? float(1.0 / (cnt * p->cout)) : 0.f;
+    HIPOK(hipMemsetAsync(p->loss.p, 0, 4, p->s));
+    HIPOK(hipMemsetAsync(p->g.p, 0, size_t(p->depth) * C * KK * 4, p->s));
+    HIPOK(hipMemsetAsync(p->gb.p, 0, size_t(p->depth) * C * 4, p->s));
+    hipLaunchKernelGGL(k_loss, dim3(g1(np * C)), dim3(256), 0, p->s, p->logits.p, p->tgt.p, p->msk.p, p->dya.p,
+                       p->loss.p, np, p->cout, inv);
+    const dim3 grid(w / PX, h, n);
+    uint16_t* dy = p->dya.p;
+    uint16_t* dx = p->dyb.p;
+    for (int L = p->depth - 1; L >= 0; --L) {
+        hipLaunchKernelGGL(k_wgrad, dim3(w / PX, (h + WG_ROWS - 1) / WG_ROWS, n), dim3(256), 0, p->s, p->act[L].p, dy,
+                           p->g.p + size_t(L) * C * KK,
+                           p->gb.p + L * C, h, w);
+        if (L > 0) {
+            // the gradient at layer L's input: the flipped-weight conv of dy, zero where that input's ReLU was off
+            hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, p->s, dy, p->wf.p + size_t(L) * C * KK,
+                               static_cast<const float*>(nullptr), dx, static_cast<float*>(nullptr), p->act[L].p, 0, 0,
+                               h, w);
+            std::swap(dy, dx);
+        }
+    }
+    HIPOK(hipGetLastError());
+    if (loss_out) {
+        HIPOK(hipMemcpyAsync(loss_out, p->loss.p, 4, hipMemcpyDeviceToHost, p->s));
+        HIPOK(hipStreamSynchronize(p->s));
+    }
+    return true;
+}
+
+// compact parameter order <-> padded device layout
+void pack_layout(const pcbprior* p, std::vector<size_t>& wi, std::vector<size_t>& bi) {
+    wi.clear();
+    bi.clear();
+    for (int L = 0; L < p->depth; ++L) {
+        for (int co = 0; co < p->lout[L]; ++co)
+            for (int d = 0; d < 9; ++d)
+                for (int ci = 0; ci < p->lin[L]; ++ci) wi.push_back(size_t(L) * C * KK + size_t(co) * KK + d * C + ci);
+        for (int co = 0; co < p->lout[L]; ++co) bi.push_back(size_t(L) * C + co);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+pcbprior* pcbprior_create(int in_ch, int width, int depth, int out_ch, uint64_t seed) {
+    g_err.clear();
+    if (in_ch < 1 || in_ch > C || width < 1 || width > C || depth < 1 || out_ch < 1 || out_ch > C) {
+        g_err = "channels must be in [1, 32]";
+        return nullptr;
+    }
+    auto* p = new pcbprior;
+    p->cin = in_ch; p->width = width; p->depth = depth; p->cout = out_ch;
+    for (int L = 0; L < depth; ++L) {
+        p->lin.push_back(L == 0 ? in_ch : width);
+        p->lout.push_back(L == depth - 1 ? out_ch : width);
+    }
+    const size_t nw = size_t(depth) * C * KK, nb = size_t(depth) * C;
+    if (hipStreamCreateWithFlags(&p->s, hipStreamNonBlocking) != hipSuccess || !p->w.need(nw) || !p->g.need(nw) ||
+        !p->mo.need(nw) || !p->ve.need(nw) || !p->bias.need(nb) || !p->gb.need(nb) || !p->mb.need(nb) ||
+        !p->vb.need(nb) || !p->wt.need(nw) || !p->wf.need(nw)) {
+        pcbprior_destroy(p);
+        return nullptr;
+    }
+    // He-uniform weights from the seed (host), biases 0; padded slots stay 0
+    std::vector<float> w(nw, 0.f), b(nb, 0.f), z(nw, 0.f), zb(nb, 0.f);
+    std::mt19937_64 rng(seed);
+    for (int L = 0; L < depth; ++L) {
+        const float lim = std::sqrt(6.f / float(9 * p->lin[L]));
+        std::uniform_real_distribution<float> u(-lim, lim);
+        for (int co = 0; co < p->lout[L]; ++co)
+            for (int d = 0; d < 9; ++d)
+                for (int ci = 0; ci < p->lin[L]; ++ci) w[size_t(L) * C * KK + size_t(co) * KK + d * C + ci] = u(rng);
+    }
+    if (!h2d(p, p->w.p, w.data(), nw * 4) || !h2d(p, p->bias.p, b.data(), nb * 4) || !h2d(p, p->mo.p, z.data(), nw * 4) ||
+        !h2d(p, p->ve.p, z.data(), nw * 4) || !h2d(p, p->mb.p, zb.data(), nb * 4) || !h2d(p, p->vb.p, zb.data(), nb * 4) ||
+        !refresh_bf16(p) || !sync(p)) {
+        g_err = "initialising the weights";
+        pcbprior_destroy(p);
+        return nullptr;
+    }
+    return p;
+}
+
+void pcbprior_destroy(pcbprior* p) {
+    if (!p) return;
+    if (p->s) (void)hipStreamDestroy(p->s);
+    delete p;
+}
+
+size_t pcbprior_num_params(const pcbprior* p) {
+    if (!p) return 0;
+    size_t n = 0;
+    for (int L = 0; L < p->depth; ++L) n += size_t(p->lout[L]) * 9 * p->lin[L] + p->lout[L];
+    return n;
+}
+
+int pcbprior_get_params(const pcbprior* p, float* out) {
+    if (!p || !out) return -1;
+    const size_t nw = size_t(p->depth) * C * KK, nb = size_t(p->depth) * C;
+    std::vector<float> w(nw), b(nb);
+    pcbprior* q = const_cast<pcbprior*>(p);
+    if (!d2h(q, w.data(), p->w.p, nw * 4) || !d2h(q, b.data(), p->bias.p, nb * 4) || !sync(q)) return -1;
+    std::vector<size_t> wi, bi;
+    pack_layout(p, wi, bi);
+    size_t o = 0, a = 0, c = 0;
+    for (int L = 0; L < p->depth; ++L) {
+        const size_t nwl = size_t(p->lout[L]) * 9 * p->lin[L];
+        for (size_t k = 0; k < nwl; ++k) out[o++] = w[wi[a++]];
+        for (int k = 0; k < p->lout[L]; ++k) out[o++] = b[bi[c++]];
+    }
+    return 0;
+}
+
+int pcbprior_set_params(pcbprior* p, const float* in) {
+    if (!p || !in) return -1;
+    const size_t nw = size_t(p->depth) * C * KK, nb = size_t(p->depth) * C;
+    std::vector<float> w(nw, 0.f), b(nb, 0.f);
+    std::vector<size_t> wi, bi;
+    pack_layout(p, wi, bi);
+    size_t o = 0, a = 0, c = 0;
+    for (int L = 0; L < p->depth; ++L) {
+        const size_t nwl = size_t(p->lout[L]) * 9 * p->lin[L];
+        for (size_t k = 0; k < nwl; ++k) w[wi[a++]] = in[o++];
+        for (int k = 0; k < p->lout[L]; ++k) b[bi[c++]] = in[o++];
+    }
+    if (!h2d(p, p->w.p, w.data(), nw * 4) || !h2d(p, p->bias.p, b.data(), nb * 4) || !refresh_bf16(p) ||
+        hipStreamSynchronize(p->s) != hipSuccess)
+        return -1;
+    return 0;
+}
+
+int pcbprior_logits(pcbprior* p, const float* x, int n, int h, int w, float* logits) {
+    g_err.clear();
+    if (!p || !x || !logits) return -1;
+    if (!forward(p, x, n, h, w)) return -1;
+    if (hipMemcpyAsync(logits, p->logits.p, p->npix * p->cout * 4, hipMemcpyDeviceToHost, p->s) != hipSuccess ||
+        hipStreamSynchronize(p->s) != hipSuccess)
+        return -1;
+    return 0;
+}
+
+int pcbprior_predict(pcbprior* p, const float* x, int n, int h, int w, float* prob) {
+    g_err.clear();
+    if (!p || !x || !prob) return -1;
+    if (!forward(p, x, n, h, w)) return -1;
+    const size_t nz = p->npix * p->cout;
+    hipLaunchKernelGGL(k_sigmoid, dim3(g1(nz)), dim3(256), 0, p->s, p->logits.p, p->logits.p, nz);
+    if (hipMemcpyAsync(prob, p->logits.p, nz * 4, hipMemcpyDeviceToHost, p->s) != hipSuccess ||
+        hipStreamSynchronize(p->s) != hipSuccess)
+        return -1;
+    return 0;
+}
+
+int pcbprior_loss_grad(pcbprior* p, const float* x, const float* target, const float* mask, int n, int h, int w,
+                       float* loss, float* grads) {
+    g_err.clear();
+    if (!p || !x || !target || !mask) return -1;
+    if (!forward(p, x, n, h, w) || !backward(p, target, mask, n, h, w, loss)) return -1;
+    if (grads) {
+        const size_t nw = size_t(p->depth) * C * KK, nb = size_t(p->depth) * C;
+        std::vector<float> g(nw), gb(nb);
+        if (!d2h(p, g.data(), p->g.p, nw * 4) || !d2h(p, gb.data(), p->gb.p, nb * 4) || !sync(p)) return -1;
+        std::vector<size_t> wi, bi;
+        pack_layout(p, wi, bi);
+        size_t o = 0, a = 0, c = 0;
+        for (int L = 0; L < p->depth; ++L) {
+            const size_t nwl = size_t(p->lout[L]) * 9 * p->lin[L];
+            for (size_t k = 0; k < nwl; ++k) grads[o++] = g[wi[a++]];
+            for (int k = 0; k < p->lout[L]; ++k) grads[o++] = gb[bi[c++]];
+        }
+    }
+    return 0;
+}
+
+int pcbprior_train_step(pcbprior* p, const float* x, const float* target, const float* mask, int n, int h, int w,
+                        float lr, float* loss) {
+    g_err.clear();
+    if (!p || !x || !target || !mask) return -1;
+    if (!forward(p, x, n, h, w) || !backward(p, target, mask, n, h, w, nullptr)) return -1;
+    ++p->step;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - std::pow(b1, float(p->step)), c2 = 1.f - std::pow(b2, float(p->step));
+    const size_t nw = size_t(p->depth) * C * KK, nb = size_t(p->depth) * C;
+    hipLaunchKernelGGL(k_adam, dim3(g1(nw)), dim3(256), 0, p->s, p->w.p, p->g.p, p->mo.p, p->ve.p, nw, lr, b1, b2,
+                       eps, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3(g1(nb)), dim3(256), 0, p->s, p->bias.p, p->gb.p, p->mb.p, p->vb.p, nb, lr, b1, b2,
+                       eps, c1, c2);
+    if (!refresh_bf16(p)) return -1;
+    if (loss && (hipMemcpyAsync(loss, p->loss.p, 4, hipMemcpyDeviceToHost, p->s) != hipSuccess)) return -1;
+    if (hipStreamSynchronize(p->s) != hipSuccess) return -1;
+    return 0;
+}
+
+int pcbprior_mfma_check(const float* a, const float* b, float* d) {
+    uint16_t ha[256], hb[256];
+    for (int i = 0; i < 256; ++i) { ha[i] = f2bf(a[i]); hb[i] = f2bf(b[i]); }
+    uint16_t *da = nullptr, *dbb = nullptr;
+    float* dd = nullptr;
+    bool ok = hipMalloc(&da, 512) == hipSuccess && hipMalloc(&dbb, 512) == hipSuccess && hipMalloc(&dd, 1024) == hipSuccess &&
+              hipMemcpy(da, ha, 512, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(dbb, hb, 512, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_mfma_check, dim3(1), dim3(64), 0, 0, da, dbb, dd);
+        ok = hipGetLastError() == hipSuccess && hipMemcpy(d, dd, 1024, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (da) (void)hipFree(da);
+    if (dbb) (void)hipFree(dbb);
+    if (dd) (void)hipFree(dd);
+    return ok ? 0 : -1;
+}
+
+const char* pcbprior_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,5 +1,5 @@
-"""The C ABI: libpcbgpu.so loads and exports every function include/pcbgpu.h declares, and the ctypes mirror of
-the structs has the C layout (offsets printed by a C program compiled here). No device calls."""
+"""The C ABI: libpcbgpu.so loads and exports every function the headers under include/ declare, and the ctypes mirror
+of the structs has the C layout (offsets printed by a C program compiled here). No device calls."""
 import ctypes as C
 import os
 import re
@@ -13,11 +13,14 @@ from pcbroute_amd.abi import Entry, Out, Query
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "pcbgpu.h")
+HEADERS = sorted(os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
 
 
 def declared():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(pcbgpu_[a-z_]+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        names |= set(re.findall(r"\b((?:pcbgpu|pcbprior)_[a-z0-9_]+)\s*\(", open(h).read()))
+    return sorted(names)
 
 
 def test_header_declares_the_reference_entry_points():
@@ -32,11 +35,12 @@ def test_library_exports_every_declared_function():
         pytest.fail("libpcbgpu.so is not built (python -m pcbroute_amd.build)")
     lib = load_library()
     out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True, check=True).stdout
-    exported = set(re.findall(r"\bT (pcbgpu_[a-z_]+)", out))
+    exported = set(re.findall(r"\bT ((?:pcbgpu|pcbprior)_[a-z0-9_]+)", out))
     missing = [n for n in declared() if n not in exported]
     assert not missing, missing
     for n in declared():
         assert getattr(lib, n) is not None
+    assert "pcbprior_train_step" in declared()
 
 
 def test_struct_layout_matches_c():
