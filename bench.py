@@ -8,6 +8,8 @@ the same searches one after another (one solve at a time) are timed beside it. P
 value: grid states solved per second (64 x 12.8M states per step / wall time of the round mode).
 roofline: the sweep kernel k_pass, algorithmic bytes = 12 B per state carried through a tile visit (int32 cost word
 read, int32 field read and write) / summed kernel time from HIP events on the launch stream.
+Tile rows, local sweeps and the delta band come from the library's autotuner, run before the timed region on a
+sample of the same workload drawn with another seed (--tune off: the library defaults or the explicit flags).
 
 --gpus N with N > 1 exits: the back end solves each search on a single device.
 """
@@ -69,6 +71,16 @@ def run(b, g, probs, mode, steps, warmup):
     return wall, st, sweeps
 
 
+def tune(b, pairs, a):
+    """The autotuner on a sample round (another seed than the timed searches); the chosen setting, or None."""
+    if a.tune == "off":
+        return None
+    t0 = time.perf_counter()
+    t = b.autotune(pairs, reps=2)
+    log(f"autotune: {t} in {time.perf_counter() - t0:.1f} s")
+    return {k: t[k] for k in ("tile_rows", "local_sweeps", "delta", "tried")}
+
+
 def round_workload(b, a):
     """256 net searches of one speculative round (2-8 pins, windows log-uniform 64..2048 cells plus a margin) on a
     2400x1800x6 board with two net classes: one round, then the same searches one at a time in submission order;
@@ -82,6 +94,7 @@ def round_workload(b, a):
     pairs = [(gs[k], p) for k, p in searches]
     states = sum(p.window[2] * p.window[3] * p.nl for _, p in searches)
     log(f"{len(pairs)} searches, {states / 1e6:.0f}M window states")
+    tuned = tune(b, [(gs[k], p) for k, p in synth.net_searches(grids, a.pairs_seed + 1000, 64)], a)
     b.round(pairs, field=False)  # warmup
     b.stats(enable=True, reset=True)
     t0 = time.perf_counter()
@@ -110,7 +123,7 @@ def round_workload(b, a):
         "unit": "searches/s", "n_gpus": 1, "higher_is_better": True, "dtype": "int32", "data": "synthetic",
         "config": {"workload": "batched speculative round: 2400x1800x6, two net classes, 256 searches of 2-8 pins, "
                                "boxes log-uniform 64..2048 cells", "board_seed": a.board_seed,
-                   "pairs_seed": a.pairs_seed, "window_states": states},
+                   "pairs_seed": a.pairs_seed, "window_states": states, "tuned": tuned},
         "round_s": round(t_round, 4), "one_at_a_time_s": round(t_one, 4), "speedup": round(t_one / t_round, 2),
         "bit_exact_with_one_at_a_time": bool(same), "reached": int(sum(x.best >= 0 for x in rr)),
         "tries_total": int(sum(x.tries for x in rr)),
@@ -143,6 +156,9 @@ def main():
     ap.add_argument("--workload", choices=("kernel", "round"), default="kernel",
                     help="kernel: the 64 whole-grid searches (the metric); round: 256 net searches of a speculative "
                          "round on a 2400x1800x6 board with two net classes, as one round and one at a time")
+    ap.add_argument("--tune", choices=("auto", "off"), default="auto",
+                    help="auto: the autotuner picks tile rows, local sweeps and delta on a sample drawn with another "
+                         "seed (the default unless --tile-rows, --local-sweeps or --delta is given)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 4 times (calibrates the HBM byte counters) and exit")
     a = ap.parse_args()
@@ -165,6 +181,8 @@ def main():
     b = Backend()
     if not b.available():
         raise SystemExit("no HIP device")
+    if a.tile_rows or a.local_sweeps or a.delta != DEFAULT_DELTA:
+        a.tune = "off"
     if a.tile_rows or a.local_sweeps:
         b.set_tile(a.tile_rows, a.local_sweeps)
     b.set_delta(a.delta)
@@ -175,8 +193,10 @@ def main():
     probs = synth.pair_problems(board, a.pairs_seed, a.pairs)
     nl, ny, nx = board[0].shape
     g = b.grid(probs[0])
+    tuned = tune(b, [(g, p) for p in synth.pair_problems(board, a.pairs_seed + 1000, 16)], a)
     rows, lsw = b.get_tile(nl)
-    log(f"board {nx}x{ny}x{nl}, {len(probs)} searches; tile rows {rows}, local sweeps {lsw}, delta {a.delta}")
+    delta = tuned["delta"] if tuned else a.delta
+    log(f"board {nx}x{ny}x{nl}, {len(probs)} searches; tile rows {rows}, local sweeps {lsw}, delta {delta}")
     modes = ("round", "single") if a.mode == "both" else (a.mode,)
     res = {m: run(b, g, probs, m, a.steps, a.warmup) for m in modes}
     g.close()
@@ -205,7 +225,7 @@ def main():
                    "grid": [nx, ny, nl], "searches_per_step": len(probs), "obstacles": 0.2, "congested": 0.3,
                    "via_cost": synth.VIA, "move_weights": [synth.ORTH, synth.DIAG], "board_seed": a.board_seed,
                    "pairs_seed": a.pairs_seed, "mode": head, "tile_rows": rows, "local_sweeps": lsw,
-                   "delta": a.delta},
+                   "delta": delta, "autotuned": tuned is not None},
         "searches_per_s": round(n_search / wall, 3),
         "ms_per_search": round(wall / n_search * 1e3, 3),
         "passes_per_search": round(sum(sweeps) / len(sweeps), 2),
@@ -223,7 +243,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "k_pass_kernel_board_2000x1600x4_round.traffic.json")
     if os.path.exists(tpath):
         t = json.load(open(tpath))
-        want = {"tile_rows": rows, "local_sweeps": lsw, "delta": a.delta, "mode": head, "pairs": len(probs)}
+        want = {"tile_rows": rows, "local_sweeps": lsw, "delta": delta, "mode": head, "pairs": len(probs)}
         if t.get("config") == want:
             line["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
             line["roofline"]["traffic_source"] = t["source"]

@@ -158,6 +158,30 @@ void pcbgpu_get_tile(int32_t nl, int32_t* tile_rows, int32_t* local_sweeps);
  * lies within delta of the least key; 0 picks 80 times the cheapest straight move weight (the default), -1 visits
  * every queued tile. Results do not depend on it. 0 or -1. */
 int pcbgpu_set_delta(int32_t delta);
+/* ---- autotuner (no counterpart in the reference). Tile rows per wavefront, local sweeps per tile visit and the
+ * delta band, picked per layer count by timing a sample round; a picked setting takes precedence over
+ * pcbgpu_set_tile / pcbgpu_set_delta for rounds and solves of that layer count. Results never depend on it. */
+typedef struct pcbgpu_tune {
+    int32_t tile_rows;     /* 16 or 32 */
+    int32_t local_sweeps;  /* 1..1024 */
+    int32_t delta;         /* 0 automatic, -1 off, else the band */
+    int32_t tried;         /* candidates timed (pcbgpu_autotune's output) */
+    double ms;             /* the sample round's time under this setting */
+} pcbgpu_tune;
+/* Times the sample round (n searches, qs[i] on grids[i], one layer count) under each candidate by coordinate
+ * descent (rows, then local sweeps, then delta, best of reps runs each), checks that every candidate returns the same
+ * best states and rectangles, installs the fastest for that layer count and copies it to best. Returns the number
+ * of candidates timed, -1 on error (the previous setting stays). Call it while no other solve runs. */
+int pcbgpu_autotune(const pcbgpu_grid* const* grids, const pcbgpu_query* const* qs, int n, int reps, pcbgpu_tune* best);
+/* the setting in force for a layer count: 1 if the autotuner (or tune_set/load) picked it, 0 if it is the default */
+int pcbgpu_tune_get(int32_t nl, pcbgpu_tune* t);
+/* install a setting for a layer count (t NULL: back to the defaults). 0 or -1. */
+int pcbgpu_tune_set(int32_t nl, const pcbgpu_tune* t);
+/* a text table, one line per tuned layer count: "layers tile_rows local_sweeps delta ms" */
+int pcbgpu_tune_save(const char* path);
+/* returns the number of settings installed, -1 if the file cannot be read */
+int pcbgpu_tune_load(const char* path);
+
 /* Kernel accounting, the counterpart of the reference's GPU_PROF timing (src/gpu.cu:491,504-516). While enabled,
  * every sweep-pass launch is bracketed by HIP events on its stream and the states it processed are counted. */
 typedef struct pcbgpu_stats {

@@ -19,8 +19,14 @@ EXPORTS = (
     "pcbgpu_backtrace", "pcbgpu_grid_create", "pcbgpu_grid_destroy", "pcbgpu_grid_upload", "pcbgpu_solve_grid",
     "pcbgpu_solve_rect_grid", "pcbgpu_round_begin", "pcbgpu_round_submit", "pcbgpu_round_submit_rect", "pcbgpu_round_flags", "pcbgpu_round_launch",
     "pcbgpu_round_result", "pcbgpu_round_end", "pcbgpu_set_tile", "pcbgpu_get_tile", "pcbgpu_set_delta", "pcbgpu_stats_enable",
-    "pcbgpu_stats_read", "pcbgpu_last_error",
+    "pcbgpu_stats_read", "pcbgpu_last_error", "pcbgpu_autotune", "pcbgpu_tune_get", "pcbgpu_tune_set",
+    "pcbgpu_tune_save", "pcbgpu_tune_load",
 )
+
+
+class Tune(C.Structure):
+    _fields_ = [("tile_rows", C.c_int32), ("local_sweeps", C.c_int32), ("delta", C.c_int32), ("tried", C.c_int32),
+                ("ms", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -78,6 +84,17 @@ def load_library(path=LIB_PATH):
     lib.pcbgpu_stats_enable.argtypes = [C.c_int]
     lib.pcbgpu_stats_read.argtypes = [C.POINTER(Stats), C.c_int]
     lib.pcbgpu_last_error.restype = C.c_char_p
+    lib.pcbgpu_autotune.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.POINTER(Query)), C.c_int, C.c_int,
+                                    C.POINTER(Tune)]
+    lib.pcbgpu_autotune.restype = C.c_int
+    lib.pcbgpu_tune_get.argtypes = [C.c_int32, C.POINTER(Tune)]
+    lib.pcbgpu_tune_get.restype = C.c_int
+    lib.pcbgpu_tune_set.argtypes = [C.c_int32, C.POINTER(Tune)]
+    lib.pcbgpu_tune_set.restype = C.c_int
+    lib.pcbgpu_tune_save.argtypes = [C.c_char_p]
+    lib.pcbgpu_tune_save.restype = C.c_int
+    lib.pcbgpu_tune_load.argtypes = [C.c_char_p]
+    lib.pcbgpu_tune_load.restype = C.c_int
     return lib
 
 
@@ -176,6 +193,49 @@ class Backend:
         r, s = C.c_int32(), C.c_int32()
         self.lib.pcbgpu_get_tile(int(nl), C.byref(r), C.byref(s))
         return r.value, s.value
+
+    @staticmethod
+    def _tune_dict(t):
+        return {k: getattr(t, k) for k, _ in Tune._fields_}
+
+    def autotune(self, pairs, reps=2):
+        """Picks tile rows, local sweeps and the delta band for the round [(grid, problem)] by timing it
+        (pcbgpu_autotune) and installs them for its layer count. Returns the chosen setting."""
+        qs = [p.query() for _, p in pairs]
+        grids = (C.c_void_p * len(pairs))(*[g.h for g, _ in pairs])
+        qp = (C.POINTER(Query) * len(pairs))(*[C.pointer(q) for q in qs])
+        t = Tune()
+        n = self.lib.pcbgpu_autotune(grids, qp, len(pairs), int(reps), C.byref(t))
+        if n < 0:
+            raise BackendError(f"autotune failed: {self.error()}")
+        return self._tune_dict(t)
+
+    def tune_get(self, nl):
+        t = Tune()
+        rc = self.lib.pcbgpu_tune_get(int(nl), C.byref(t))
+        if rc < 0:
+            raise BackendError("bad layer count")
+        return self._tune_dict(t), bool(rc)
+
+    def tune_set(self, nl, tile_rows=None, local_sweeps=4, delta=0):
+        """Installs a setting for a layer count (tile_rows None: back to the defaults)."""
+        if tile_rows is None:
+            rc = self.lib.pcbgpu_tune_set(int(nl), None)
+        else:
+            t = Tune(int(tile_rows), int(local_sweeps), int(delta), 0, 0.0)
+            rc = self.lib.pcbgpu_tune_set(int(nl), C.byref(t))
+        if rc != 0:
+            raise BackendError("bad tune setting")
+
+    def tune_save(self, path):
+        if self.lib.pcbgpu_tune_save(os.fsencode(path)) != 0:
+            raise BackendError(self.error())
+
+    def tune_load(self, path):
+        n = self.lib.pcbgpu_tune_load(os.fsencode(path))
+        if n < 0:
+            raise BackendError(self.error())
+        return n
 
     def _finish(self, out, rc, copy_field=True):
         if rc < 0 and rc != ERR_BUSY:

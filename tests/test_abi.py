@@ -65,3 +65,29 @@ def test_struct_layout_matches_c():
         assert int(got[f"{s} size"]) == C.sizeof(cls), s
         for f in fields[s]:
             assert int(got[f"{s}.{f}"]) == getattr(cls, f).offset, (s, f)
+
+
+def test_tune_table_set_save_load(tmp_path):
+    """The autotuner's table (host only: no device call): install, read back, save, clear, load."""
+    from pcbroute_amd import Backend
+    b = Backend()
+    try:
+        b.tune_set(4, 16, 3, -1)
+        t, tuned = b.tune_get(4)
+        assert tuned and (t["tile_rows"], t["local_sweeps"], t["delta"]) == (16, 3, -1)
+        f = tmp_path / "tune.txt"
+        b.tune_save(str(f))
+        assert "4 16 3 -1" in f.read_text()
+        b.tune_set(4, None)
+        assert b.tune_get(4)[1] is False
+        assert b.tune_load(str(f)) == 1
+        t, tuned = b.tune_get(4)
+        assert tuned and (t["tile_rows"], t["local_sweeps"], t["delta"]) == (16, 3, -1)
+        with pytest.raises(Exception):
+            b.tune_set(4, 24, 3, 0)  # rows must be 16 or 32
+        with pytest.raises(Exception):
+            b.tune_set(4, 16, 0, 0)  # at least one local sweep
+        with pytest.raises(Exception):
+            b.tune_get(9)
+    finally:
+        b.tune_set(4, None)

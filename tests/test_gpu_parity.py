@@ -214,3 +214,26 @@ def test_cpp_gpu_h_shim():
     _say("cpp shim")
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout + r.stderr
+
+
+def test_autotune_picks_a_setting_and_keeps_results(backend, oracle):
+    """The autotuner times a sample round under its candidates, installs the fastest for the layer count, and the
+    round's results stay those of the oracle under the picked setting."""
+    nl = 3
+    probs = [random_case(1300 + k, nx=160, ny=120, nl=nl, sym=k % 2 == 0) for k in range(6)]
+    g = backend.grid(probs[0])
+    try:
+        pairs = [(g, _on_grid(p, probs[0])) for p in probs]
+        _say("autotune", len(pairs))
+        t = backend.autotune(pairs, reps=1)
+        assert t["tried"] >= 2 and t["ms"] > 0
+        assert t["tile_rows"] in (16, 32) and 1 <= t["local_sweeps"] <= 8 and t["delta"] >= -1
+        got, tuned = backend.tune_get(nl)
+        assert tuned and (got["tile_rows"], got["local_sweeps"], got["delta"]) == (t["tile_rows"], t["local_sweeps"], t["delta"])
+        assert backend.get_tile(nl) == (t["tile_rows"], t["local_sweeps"])
+        for (_, p), r in zip(pairs, backend.round(pairs)):
+            _same(r, oracle.solve(p))
+    finally:
+        backend.tune_set(nl, None)
+        g.close()
+    assert backend.tune_get(nl)[1] is False
