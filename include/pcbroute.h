@@ -1,0 +1,110 @@
+/* pcbroute.h - C ABI of the PathFinder router that drives the MI355X search back end (pcbgpu.h).
+ *
+ * The caller of the back end: it restates the parts of pcbroute's router that build and consume GPU searches —
+ * per-class clearance grids and occupancy stamps (src/router.cpp:383-420, 1892-1963), negotiated congestion rounds
+ * with rip-up (src/router.cpp:2058-2300), the query build with starts and targets (:1525-1584), the backtrace into
+ * the route (:1602-1679), read-block validation of speculative searches (:2119-2242) and the learned corridor prior
+ * (:1216-1223, src/mlinfer.cpp:1414-1455). Boards come from a seeded synthetic generator (footprints SOIC-16,
+ * 0805, QFP-64; nets of 2-6 pins), not from DSN files. Every search runs on the GPU; there is no CPU search.
+ */
+#ifndef PCBROUTE_H
+#define PCBROUTE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pcbgpu.h"
+#include "pcbprior.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCBR_MAX_CLASSES 4
+
+/* A seeded synthetic board (sizes in mm, grid pitch in um). */
+typedef struct pcbr_board_spec {
+    double width_mm, height_mm, cell_um;
+    int32_t layers;
+    int32_t n_soic16, n_0805, n_qfp64;    /* parts placed uniformly without overlap */
+    int32_t n_nets;                       /* nets of 2-6 pins (geometric, p = 0.5), pins drawn among the pads */
+    int32_t n_classes;                    /* net classes 1..PCBR_MAX_CLASSES, assigned round robin */
+    double track_mm[PCBR_MAX_CLASSES], clear_mm[PCBR_MAX_CLASSES], via_mm[PCBR_MAX_CLASSES];
+    double edge_mm;                       /* copper to board edge */
+    uint64_t seed;
+} pcbr_board_spec;
+
+typedef struct pcbr_board pcbr_board;
+pcbr_board* pcbr_board_generate(const pcbr_board_spec* spec);
+void pcbr_board_free(pcbr_board* b);
+
+typedef struct pcbr_board_info {
+    int32_t nx, ny, nl, n_pads, n_nets, n_pins, n_connections, n_parts;
+    uint64_t hash;  /* FNV-1a over the pads and nets: the same seed gives the same board */
+} pcbr_board_info;
+int pcbr_board_get_info(const pcbr_board* b, pcbr_board_info* info);
+/* pads as rows of 6 int32: x0, y0, x1, y1 (cells, half open), layer (-1: every layer), net (-1: none) */
+int pcbr_board_pads(const pcbr_board* b, int32_t* out, size_t cap);
+
+typedef struct pcbr_options {
+    int32_t max_iters;        /* negotiation rounds before legalizing (src/router.h:38) */
+    int32_t patience;         /* stop after this many rounds without a 10 % improvement */
+    double pres0, pres_growth, hist_inc;
+    double window_mm;         /* search window margin around a net's pins (src/router.h:15) */
+    double via_cost_mm;       /* a via costs as much as this much straight track (src/router.h:16) */
+    double wrong_way;         /* cost factor against a layer's preferred direction (src/router.h:24) */
+    int32_t wave;             /* 0: one net at a time; > 0: up to this many nets routed together, committed in
+                                 order with read-block validation (the result equals one at a time) */
+    /* learned prior: applied in the first round when a network is attached (pcbr_set_prior) */
+    double ml_weight;         /* a step costs up to 1 + this where the prior says no (src/router.h:59) */
+    double ml_sat;            /* prior at or above this: no extra cost (src/router.h:60) */
+    double ml_corridor;       /* search only where the prior is at least this, falling back to the plain search */
+    double ml_margin_mm;      /* the learned window: the corridor's bounding box plus this */
+    int32_t legalize;         /* after negotiation, reroute what still conflicts with others' copper as obstacles */
+    int32_t verbose;
+} pcbr_options;
+void pcbr_default_options(pcbr_options* o);
+
+typedef struct pcbr_result {
+    int32_t nets, connections, opens, violations, conflict_nets, iterations;
+    int64_t searches, search_fallbacks, rounds, revalidated, length_cells, vias;
+    uint64_t tile_visits;
+    double route_s, search_s, prior_s, upload_s, backtrace_s;
+    uint64_t route_hash;      /* FNV-1a over every net's routed cells in order: equal routes, equal hash */
+} pcbr_result;
+
+typedef struct pcbr_router pcbr_router;
+pcbr_router* pcbr_create(const pcbr_board* b, const pcbr_options* o);
+void pcbr_destroy(pcbr_router* r);
+/* routes the board; 0 or -1 (pcbr_last_error) */
+int pcbr_route(pcbr_router* r, pcbr_result* res);
+/* the independent check of the committed routes (the counterpart of --eval and check_routes,
+ * src/eval.cpp:67-220, src/router.cpp:2864-2936): copper connectivity per net and clearance between nets'
+ * tracks, vias and pads, recomputed from the routes alone */
+int pcbr_eval(pcbr_router* r, int32_t* opens, int32_t* violations);
+/* a net's route as rows of 3 int32 (layer, i, j) in path order, paths separated by a row (-1, -1, -1) */
+int64_t pcbr_net_route(const pcbr_router* r, int32_t net, int32_t* out, size_t cap_rows);
+
+/* Called after every search with its query and canonical result (field included), for checking. */
+typedef void (*pcbr_search_cb)(void* user, const pcbgpu_query* q, const pcbgpu_out* out);
+void pcbr_set_search_callback(pcbr_router* r, pcbr_search_cb cb, void* user);
+
+/* ---- the learned corridor prior (replaces the RouteNet call, src/mlinfer.cpp:834-856) */
+#define PCBR_FEAT_PER_LAYER 5  /* obstacles, others' occupancy, history, this net's pads, other nets' pads */
+#define PCBR_PIX 4             /* a prior pixel is 4 x 4 grid cells */
+/* attach a network (in_ch = PCBR_FEAT_PER_LAYER * layers, out_ch = layers); NULL detaches */
+int pcbr_set_prior(pcbr_router* r, pcbprior* net);
+/* Training samples: when enabled, the first route of every net in the first round records its input features
+ * (the board as that net's search saw it) over a crop around its pins; after pcbr_route, the net's final route is
+ * the target. Crops are ch x cw pixels (cw a multiple of 64), placed around the pins' box and inside the board. */
+int pcbr_record_samples(pcbr_router* r, int32_t ch, int32_t cw);
+int32_t pcbr_num_samples(const pcbr_router* r);
+/* sample k: x (ch, cw, in_ch), target (ch, cw, layers) in {0, 1}, mask (ch, cw) */
+int pcbr_get_sample(const pcbr_router* r, int32_t k, float* x, float* target, float* mask);
+
+const char* pcbr_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

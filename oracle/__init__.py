@@ -16,6 +16,26 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 SRC = os.path.join(HERE, "pcb_oracle.c")
 
 
+ROUTE_LIB = os.path.join(HERE, "libroute_oracle.so")
+
+
+def build_route_oracle(verbose=False):
+    """The router (pcbroute_amd/csrc/router.cpp) over the oracle's search: oracle/libroute_oracle.so."""
+    root = os.path.dirname(HERE)
+    obj = os.path.join(HERE, "pcb_oracle_route.o")
+    cmds = [["gcc", "-O2", "-std=c11", "-fPIC", "-c", "-o", obj, SRC],
+            ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", ROUTE_LIB + ".tmp",
+             os.path.join(root, "pcbroute_amd", "csrc", "router.cpp"), os.path.join(HERE, "route_on_oracle.cpp"), obj,
+             "-lpthread"]]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    os.replace(ROUTE_LIB + ".tmp", ROUTE_LIB)
+    os.remove(obj)
+    return ROUTE_LIB
+
+
 def build(verbose=False):
     tmp = LIB_PATH + ".tmp"
     cmd = ["gcc", "-O2", "-std=c11", "-Wall", "-Wextra", "-fPIC", "-shared", "-o", tmp, SRC, "-lpthread"]

@@ -1,0 +1,79 @@
+"""The router on the HIP back end: the same routes, bit for bit, as over the oracle's exact search (the committed
+fixture), each search's canonical field equal to the oracle's, waves equal to one net at a time, and a clean
+independent check of the routes on a full-size 4-layer board."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from pcbroute_amd import router as R
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "route_golden.json")
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("kind,seed", [("tiny", 1), ("tiny", 2), ("tiny", 3), ("c1", 1)])
+def test_routes_equal_the_oracle_routes(kind, seed):
+    g = [x for x in json.load(open(GOLDEN)) if x["kind"] == kind and x["seed"] == seed][0]
+    _say("route", kind, seed)
+    b = R.Board(R.spec(kind, seed=seed))
+    assert b.info.hash == g["board_hash"]
+    r = R.Router(b)
+    res = r.route()
+    _say(res)
+    for k in ("route_hash", "opens", "violations", "searches", "length_cells", "vias", "iterations"):
+        assert res[k] == g[k], k
+    assert r.eval() == (0, 0)
+
+
+def test_every_search_field_equals_the_oracle(oracle):
+    from pcbroute_amd.abi import Out
+    b = R.Board(R.spec("tiny", seed=2))
+    r = R.Router(b)
+    seen = []
+
+    def check(qp, op):
+        o = op.contents
+        ref = Out()
+        assert oracle.lib.orc_solve(qp, C.byref(ref), 1 << 31) >= 0
+        try:
+            assert (o.rx0, o.ry0, o.rw, o.rh, o.best) == (ref.rx0, ref.ry0, ref.rw, ref.rh, ref.best)
+            n = o.rw * o.rh * o.nl
+            a = np.ctypeslib.as_array(o.d, (n,))
+            z = np.ctypeslib.as_array(ref.d, (n,))
+            seen.append(bool(np.array_equal(a, z)))
+        finally:
+            oracle.lib.orc_out_free(C.byref(ref))
+
+    r.on_search(check)
+    _say("field check")
+    res = r.route()
+    r.on_search(None)
+    assert len(seen) == res["searches"] and all(seen)
+
+
+def test_waves_equal_one_at_a_time_on_the_gpu():
+    g = [x for x in json.load(open(GOLDEN)) if x["kind"] == "c1"][0]
+    b = R.Board(R.spec("c1", seed=1))
+    _say("waves c1")
+    res = R.Router(b, R.default_options(wave=16)).route()
+    _say(res)
+    assert res["route_hash"] == g["route_hash"]
+    assert res["rounds"] < res["searches"]
+
+
+def test_full_size_board_routes_clean():
+    b = R.Board(R.spec("c4", seed=101))
+    _say("c4 board", b.info.n_nets, b.info.n_connections)
+    r = R.Router(b, R.default_options(wave=32))
+    res = r.route()
+    _say(res)
+    assert res["violations"] == 0 and res["conflict_nets"] == 0
+    assert res["opens"] <= 0.05 * res["connections"]
+    assert r.eval() == (res["opens"], 0)
