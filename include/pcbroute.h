@@ -54,7 +54,11 @@ typedef struct pcbr_options {
     double via_cost_mm;       /* a via costs as much as this much straight track (src/router.h:16) */
     double wrong_way;         /* cost factor against a layer's preferred direction (src/router.h:24) */
     int32_t wave;             /* 0: one net at a time; > 0: up to this many nets routed together, committed in
-                                 order with read-block validation (the result equals one at a time) */
+                                 order with read-block validation (the result equals one at a time); < 0: each
+                                 group of the order (group_mm) routed together, at most -wave nets at once */
+    double group_mm;          /* the routing order: maximal groups of nets whose pin boxes, grown by this, do not
+                                 overlap, group after group (src/router.cpp:2128-2137); < 0: each box grown by
+                                 -group_mm times its longer side plus 1 mm */
     /* learned prior: applied in the first round when a network is attached (pcbr_set_prior) */
     double ml_weight;         /* a step costs up to 1 + this where the prior says no (src/router.h:59) */
     double ml_sat;            /* prior at or above this: no extra cost (src/router.h:60) */
@@ -91,8 +95,9 @@ void pcbr_set_search_callback(pcbr_router* r, pcbr_search_cb cb, void* user);
 
 /* ---- the learned corridor prior (replaces the RouteNet call, src/mlinfer.cpp:834-856) */
 #define PCBR_FEAT_PER_LAYER 5  /* obstacles, others' occupancy, history, this net's pads, other nets' pads */
+#define PCBR_FEAT_NET 2        /* then, once per pixel: closeness to the net's pin spanning tree, its pins' box */
 #define PCBR_PIX 4             /* a prior pixel is 4 x 4 grid cells */
-/* attach a network (in_ch = PCBR_FEAT_PER_LAYER * layers, out_ch = layers); NULL detaches */
+/* attach a network (in_ch = PCBR_FEAT_PER_LAYER * layers + PCBR_FEAT_NET, out_ch = layers); NULL detaches */
 int pcbr_set_prior(pcbr_router* r, pcbprior* net);
 /* Training samples: when enabled, the first route of every net in the first round records its input features
  * (the board as that net's search saw it) over a crop around its pins; after pcbr_route, the net's final route is

@@ -1,0 +1,173 @@
+"""Trains and evaluates the learned corridor prior on seeded synthetic 4-layer boards.
+
+Training data: the plain method (the router with no prior) routes each training board; every net's first search
+records its input features (the board as the net's search saw it) over a crop around its pins, and the net's
+committed route is the target (src/mlinfer.cpp:1414-1455 makes the reference's prior from RouteNet's output the
+same way). The network (6 conv3x3 layers, 32 channels, BF16 matrix-core kernels with FP32 accumulation) trains on
+the GPU with Adam. Evaluation: held-out seeds routed plainly one net at a time and batched with the prior; open
+connections, violations and wall time side by side.
+
+    python -m pcbroute_amd.learn --train 64 --eval 16 --out pcbroute_amd/weights/corridor_c4.npy
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import prior as P
+from . import router as R
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(HERE, "weights", "corridor_c4.npy")
+CROP = (96, 128)  # prior pixels (4 x 4 cells): 19.2 x 25.6 mm
+TRAIN_SEED0, EVAL_SEED0 = 1000, 5000
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_net(nl=4, seed=0):
+    return P.PriorNet(R.FEAT_PER_LAYER * nl + R.FEAT_NET, 32, 6, nl, seed=seed)
+
+
+def load_net(path=WEIGHTS, nl=4):
+    net = make_net(nl)
+    net.set_params(np.load(path, allow_pickle=False))
+    return net
+
+
+def collect(seed, wave=0):
+    """Routes board `seed` plainly; returns its samples (x, target, mask) and the route figures."""
+    b = R.Board(R.spec("c4", seed=seed))
+    r = R.Router(b, R.default_options(wave=wave))
+    r.record_samples(*CROP)
+    res = r.route()
+    x, t, m = r.samples()
+    r.close()
+    b.close()
+    return (x, t, m), res
+
+
+def train(net, data, epochs=4, batch=16, lr=1e-3, seed=0):
+    x, t, m = data
+    rng = np.random.default_rng(seed)
+    n = len(x)
+    losses = []
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        el = []
+        for k in range(0, n - batch + 1, batch):
+            idx = order[k:k + batch]
+            el.append(net.train_step(x[idx], t[idx], m[idx], lr=lr))
+        losses.append(float(np.mean(el)))
+        log(f"epoch {ep + 1}/{epochs}: loss {losses[-1]:.4f}")
+    return losses
+
+
+def corridor_stats(net, data, thr=0.05):
+    """How the prediction covers the routes on held-out samples: recall of route pixels above thr (with the 2-pixel
+    growth the router applies) and the fraction of the crop kept."""
+    x, t, m = data
+    p = np.concatenate([net.predict(x[k:k + 64]) for k in range(0, len(x), 64)])
+    from numpy.lib.stride_tricks import sliding_window_view as sw
+    pad = np.pad(p, ((0, 0), (2, 2), (2, 2), (0, 0)))
+    g = sw(pad, (5, 5), axis=(1, 2)).max(axis=(-1, -2))
+    keep = (g >= thr) & (m[..., None] > 0)
+    route = (t > 0) & (m[..., None] > 0)
+    recall = float((keep & route).sum() / max(1, route.sum()))
+    kept = float(keep.sum() / max(1, (m[..., None] > 0).sum() * p.shape[-1]))
+    return recall, kept
+
+
+CONFIGS = {  # name: (wave, group_mm, prior)
+    "plain": (0, 1.0, False),
+    "batched": (-64, 4.0, False),
+    "learned": (0, 1.0, True),
+    "batched_learned": (-64, 4.0, True),
+}
+FIELDS = ("opens", "violations", "route_s", "search_s", "prior_s", "searches", "rounds", "revalidated", "iterations",
+          "search_fallbacks", "length_cells", "vias")
+
+
+def evaluate(net, seeds, configs=CONFIGS):
+    """Each held-out board routed under each configuration (one net at a time or batched; plain or learned)."""
+    rows = []
+    for s in seeds:
+        b = R.Board(R.spec("c4", seed=s))
+        row = {"seed": s, "nets": b.info.n_nets, "connections": b.info.n_connections}
+        for name, (wave, group, use) in configs.items():
+            r = R.Router(b, R.default_options(wave=wave, group_mm=group))
+            if use:
+                r.set_prior(net)
+            res = r.route()
+            r.close()
+            row[name] = {k: res[k] for k in FIELDS}
+        rows.append(row)
+        log(f"seed {s}: " + "; ".join(f"{k} {row[k]['route_s']:.2f} s {row[k]['opens']} open "
+                                      f"{row[k]['violations']} viol" for k in configs))
+    return rows
+
+
+def summary(rows):
+    names = [k for k in rows[0] if isinstance(rows[0][k], dict)]
+    tot = {k: {"route_s": 0.0, "opens": 0, "violations": 0, "search_fallbacks": 0} for k in names}
+    for r in rows:
+        for k in names:
+            for f in tot[k]:
+                tot[k][f] += r[k][f]
+    for k in names:
+        tot[k + "_over_plain_time"] = tot[k]["route_s"] / max(1e-9, tot["plain"]["route_s"])
+    return tot
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=64, help="training boards")
+    ap.add_argument("--eval", type=int, default=16, help="held-out boards")
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--report", default=None, help="JSON report path")
+    ap.add_argument("--load", default=None, help="skip training: evaluate these weights")
+    a = ap.parse_args()
+    t0 = time.time()
+    rep = {"crop": CROP}
+    if a.load:
+        net = load_net(a.load)
+    else:
+        xs, ts, ms, figs = [], [], [], []
+        for k in range(a.train):
+            (x, t, m), res = collect(TRAIN_SEED0 + k)
+            xs.append(x); ts.append(t); ms.append(m)
+            figs.append({k2: res[k2] for k2 in ("nets", "connections", "opens", "route_s")})
+            log(f"board {k + 1}/{a.train}: {len(x)} samples, {res['opens']} open, {res['route_s']:.2f} s")
+        data = (np.concatenate(xs), np.concatenate(ts), np.concatenate(ms))
+        nval = max(1, len(data[0]) // 10)
+        val = tuple(d[:nval] for d in data)
+        tr = tuple(d[nval:] for d in data)
+        net = make_net()
+        t1 = time.time()
+        rep["losses"] = train(net, tr, epochs=a.epochs, lr=a.lr)
+        rep["train_s"] = time.time() - t1
+        rep["samples"] = int(len(tr[0]))
+        rep["val_recall"], rep["val_kept"] = corridor_stats(net, val)
+        log(f"validation: route recall {rep['val_recall']:.3f} at 0.05, crop kept {rep['val_kept']:.3f}")
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+        np.save(a.out, net.get_params())
+        rep["collect_boards"] = figs
+    rows = evaluate(net, [EVAL_SEED0 + k for k in range(a.eval)])
+    rep["eval"] = rows
+    rep["summary"] = summary(rows)
+    rep["total_s"] = time.time() - t0
+    log(json.dumps(rep["summary"]))
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(rep, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

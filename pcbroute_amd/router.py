@@ -11,6 +11,7 @@ from .abi import Out, Query
 
 MAX_CLASSES = 4
 FEAT_PER_LAYER = 5
+FEAT_NET = 2
 PIX = 4
 
 
@@ -30,7 +31,7 @@ class BoardInfo(C.Structure):
 class Options(C.Structure):
     _fields_ = [("max_iters", C.c_int32), ("patience", C.c_int32), ("pres0", C.c_double),
                 ("pres_growth", C.c_double), ("hist_inc", C.c_double), ("window_mm", C.c_double),
-                ("via_cost_mm", C.c_double), ("wrong_way", C.c_double), ("wave", C.c_int32),
+                ("via_cost_mm", C.c_double), ("wrong_way", C.c_double), ("wave", C.c_int32), ("group_mm", C.c_double),
                 ("ml_weight", C.c_double), ("ml_sat", C.c_double), ("ml_corridor", C.c_double),
                 ("ml_margin_mm", C.c_double), ("legalize", C.c_int32), ("verbose", C.c_int32)]
 
@@ -175,7 +176,7 @@ class Router:
         self.lib.pcbr_set_search_callback(self.h, self._cb, None)
 
     def set_prior(self, net):
-        """Attach a pcbroute_amd.prior.PriorNet (in_ch = 5 x layers, out_ch = layers), or None."""
+        """Attach a pcbroute_amd.prior.PriorNet (in_ch = 5 x layers + 2, out_ch = layers), or None."""
         self._prior = net
         self.lib.pcbr_set_prior(self.h, net.h if net is not None else None)
 
@@ -203,11 +204,11 @@ class Router:
         return a
 
     def samples(self):
-        """The recorded training samples: x (n, ch, cw, 5 nl), target (n, ch, cw, nl), mask (n, ch, cw)."""
+        """The recorded training samples: x (n, ch, cw, 5 nl + 2), target (n, ch, cw, nl), mask (n, ch, cw)."""
         n = self.lib.pcbr_num_samples(self.h)
         ch, cw = self._crop
         nl = self.board.info.nl
-        x = np.zeros((n, ch, cw, FEAT_PER_LAYER * nl), np.float32)
+        x = np.zeros((n, ch, cw, FEAT_PER_LAYER * nl + FEAT_NET), np.float32)
         t = np.zeros((n, ch, cw, nl), np.float32)
         m = np.zeros((n, ch, cw), np.float32)
         F = C.POINTER(C.c_float)

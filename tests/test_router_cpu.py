@@ -82,3 +82,28 @@ def test_routes_are_connected_paths(lib):
             l, i, j = p[-1]
             on = [(x0 <= i < x1 and y0 <= j < y1 and (pl < 0 or pl == l)) for x0, y0, x1, y1, pl, pn in pads if pn == n]
             assert any(on)
+
+
+def test_group_waves_equal_one_at_a_time(lib):
+    b = R.Board(R.spec("tiny", seed=2), lib_path=lib)
+    seq = R.Router(b, R.default_options(R.library(lib), group_mm=3.0), lib_path=lib).route()
+    grp = R.Router(b, R.default_options(R.library(lib), group_mm=3.0, wave=-8), lib_path=lib).route()
+    assert grp["route_hash"] == seq["route_hash"]
+
+
+def test_training_samples(lib):
+    b = R.Board(R.spec("tiny", seed=1), lib_path=lib)
+    r = R.Router(b, R.default_options(R.library(lib)), lib_path=lib)
+    r.record_samples(32, 64)
+    r.route()
+    x, t, m = r.samples()
+    nl = b.info.nl
+    assert x.shape == (b.info.n_nets, 32, 64, R.FEAT_PER_LAYER * nl + R.FEAT_NET) and t.shape == (b.info.n_nets, 32, 64, nl)
+    assert x.min() >= 0 and x.max() <= 1 and set(np.unique(t)) <= {0.0, 1.0}
+    assert np.all(m.sum(axis=(1, 2)) > 0)
+    for k in range(len(x)):
+        own_pads = x[k, :, :, 3::R.FEAT_PER_LAYER][..., :nl].max(axis=-1)  # this net's pad cells, any layer
+        assert own_pads.max() > 0
+        mst = x[k, :, :, R.FEAT_PER_LAYER * nl]
+        assert mst[own_pads > 0].max() > 0.8                      # the spanning tree passes through the pins
+        assert t[k].sum() > 0 and (t[k].max(axis=-1) * mst).sum() > 0  # the route lies along it
