@@ -74,6 +74,7 @@ typedef struct pcbr_result {
     int64_t searches, search_fallbacks, rounds, revalidated, length_cells, vias;
     uint64_t tile_visits;
     double route_s, search_s, prior_s, upload_s, backtrace_s;
+    double host_s[4];         /* host time: cost rebuilds, stamps, read sets, query builds */
     uint64_t route_hash;      /* FNV-1a over every net's routed cells in order: equal routes, equal hash */
 } pcbr_result;
 

@@ -43,7 +43,7 @@ class Result(C.Structure):
                                          "vias")] + \
                [("tile_visits", C.c_uint64)] + \
                [(k, C.c_double) for k in ("route_s", "search_s", "prior_s", "upload_s", "backtrace_s")] + \
-               [("route_hash", C.c_uint64)]
+               [("host_s", C.c_double * 4), ("route_hash", C.c_uint64)]
 
 
 SEARCH_CB = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(Query), C.POINTER(Out))
@@ -189,7 +189,9 @@ class Router:
         r = Result()
         if self.lib.pcbr_route(self.h, C.byref(r)) != 0:
             raise BackendError(self.lib.pcbr_last_error().decode())
-        return {k: getattr(r, k) for k, _ in Result._fields_}
+        out = {k: getattr(r, k) for k, _ in Result._fields_}
+        out["host_s"] = dict(zip(("rebuild", "stamp", "reads", "query"), list(r.host_s)))
+        return out
 
     def eval(self):
         o, v = C.c_int32(), C.c_int32()

@@ -77,3 +77,25 @@ def test_full_size_board_routes_clean():
     assert res["violations"] == 0 and res["conflict_nets"] == 0
     assert res["opens"] <= 0.05 * res["connections"]
     assert r.eval() == (res["opens"], 0)
+
+
+def test_learned_prior_gains_over_plain():
+    """The committed network (trained on routes the plain method committed on boards 1000-1063) against the plain
+    method on held-out boards: no violations either way, fewer open connections with the prior."""
+    from pcbroute_amd import learn
+    net = learn.load_net()
+    tot = {"plain": 0, "learned": 0}
+    for seed in (5000, 5001):
+        b = R.Board(R.spec("c4", seed=seed))
+        _say("learned vs plain", seed)
+        plain = R.Router(b).route()
+        r = R.Router(b)
+        r.set_prior(net)
+        learned = r.route()
+        _say(seed, "plain", plain["opens"], plain["route_s"], "learned", learned["opens"], learned["route_s"],
+             learned["search_fallbacks"])
+        assert plain["violations"] == 0 and learned["violations"] == 0
+        assert learned["search_fallbacks"] <= 0.01 * learned["searches"]
+        tot["plain"] += plain["opens"]
+        tot["learned"] += learned["opens"]
+    assert tot["learned"] < tot["plain"]
