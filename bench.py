@@ -132,6 +132,79 @@ def round_workload(b, a):
     }), flush=True)
 
 
+PRIOR_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix cores
+
+
+def prior_workload(b, a):
+    """The learned prior's training and inference on full 4-layer 60x50 mm boards: the 4x-downsampled grid is
+    300 x 250 pixels, padded to 320 x 256; 22 input channels, 6 conv3x3 layers of 32 channels, 4 outputs. The
+    training set stays in HBM; a step is one Adam step on a batch of 8 boards (forward, backward, update)."""
+    import numpy as np
+    from pcbroute_amd import prior as P
+    h, w, cin, cout, batch, nds = 256, 320, 22, 4, 8, 16
+    rng = np.random.default_rng(a.board_seed)
+    x = rng.random((nds, h, w, cin), dtype=np.float32)
+    t = (rng.random((nds, h, w, cout)) < 0.05).astype(np.float32)
+    m = np.zeros((nds, h, w), np.float32)
+    m[:, :250, :300] = 1
+    net = P.PriorNet(cin, 32, 6, cout, seed=1)
+    net.dataset(x, t, m)
+    batches = [rng.integers(0, nds, batch).astype(np.int32) for _ in range(a.steps + a.warmup)]
+    for k in range(a.warmup):
+        net.train_resident(batches[k], loss=False)
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        net.train_resident(batches[a.warmup + k], loss=False)
+    wall = time.perf_counter() - t0
+    log("prior: training timed")
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        net.forward_resident(batches[k], logits=False)
+    wall_inf = time.perf_counter() - t0
+    net.stats(enable=True, reset=True)
+    for k in range(a.steps):
+        net.train_resident(batches[k], loss=False)
+    st = net.stats(enable=False, reset=True)
+    log("prior: kernel accounting done")
+    px = float(batch) * h * w
+    real = 250 * 300 / (h * w)
+    step_flop = st["conv_flop"] / a.steps + st["wgrad_flop"] / a.steps
+    kern = {"k_conv": (st["conv_flop"], st["conv_ms"], st["conv_launches"]),
+            "k_wgrad": (st["wgrad_flop"], st["wgrad_ms"], st["wgrad_launches"])}
+    dom = max(kern, key=lambda k: kern[k][1])
+    fl, ms, nlaunch = kern[dom]
+    achieved = fl / (ms / 1e3) / 1e12
+    line = {
+        "metric": "prior training boards/s", "value": round(a.steps * batch / wall, 3), "unit": "boards/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic",
+        "config": {"workload": "corridor prior: 6 conv3x3 layers 22->32x5->4 on 4-layer 60x50 mm boards "
+                               "(300x250 prior pixels padded to 320x256), batch 8, Adam, training set resident in HBM",
+                   "batch": batch, "pixels": [h, w], "channels": [cin, 32, cout]},
+        "inference_boards_per_s": round(a.steps * batch / wall_inf, 3),
+        "train_tflops": round(step_flop / (wall / a.steps) / 1e12, 2),
+        "useful_fraction_of_pixels": round(real, 3),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": PRIOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PRIOR_PEAK_TFLOPS, 5), "traffic": None, "kernel": dom,
+                     "launches": nlaunch, "avg_launch_us": round(ms * 1e3 / max(1, nlaunch), 2),
+                     "flop_per_launch": round(fl / max(1, nlaunch))},
+        "kernels": {k: {"tflops": round(v[0] / (v[1] / 1e3) / 1e12, 2), "ms": round(v[1], 3), "launches": v[2]}
+                    for k, v in kern.items()},
+    }
+    if not a.no_cpu:
+        from oracle import prior_ref as R
+        params = net.unpack(net.get_params())
+        t0 = time.perf_counter()
+        R.loss_grad(params, x[:1], t[:1], m[:1], quant=False)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(1 / dt, 4), "unit": "boards/s", "cores": 1, "kind": "port",
+                                "sample": f"one training step (forward + backward, float64 numpy) on 1 board of "
+                                          f"320x256 pixels: {dt:.2f} s"}
+    net.close()
+    print(json.dumps(line), flush=True)
+
+
 def synth_problem_of(grid):
     """A Problem that only carries a class's maps (to create its device-resident grid)."""
     from pcbroute_amd import Problem
@@ -153,9 +226,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=("both", "single", "round"), default="both")
-    ap.add_argument("--workload", choices=("kernel", "round"), default="kernel",
+    ap.add_argument("--workload", choices=("kernel", "round", "prior"), default="kernel",
                     help="kernel: the 64 whole-grid searches (the metric); round: 256 net searches of a speculative "
-                         "round on a 2400x1800x6 board with two net classes, as one round and one at a time")
+                         "round on a 2400x1800x6 board with two net classes, as one round and one at a time; "
+                         "prior: the corridor CNN's training and inference on full 4-layer boards")
     ap.add_argument("--tune", choices=("auto", "off"), default="auto",
                     help="auto: the autotuner picks tile rows, local sweeps and delta on a sample drawn with another "
                          "seed (the default unless --tile-rows, --local-sweeps or --delta is given)")
@@ -188,6 +262,8 @@ def main():
     b.set_delta(a.delta)
     if a.workload == "round":
         return round_workload(b, a)
+    if a.workload == "prior":
+        return prior_workload(b, a)
     log("building the board")
     board = synth.kernel_board(a.board_seed)
     probs = synth.pair_problems(board, a.pairs_seed, a.pairs)

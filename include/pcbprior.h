@@ -47,6 +47,25 @@ int pcbprior_loss_grad(pcbprior* p, const float* x, const float* target, const f
 int pcbprior_train_step(pcbprior* p, const float* x, const float* target, const float* mask, int n, int h, int w,
                         float lr, float* loss);
 
+/* ---- a training set resident in HBM (no counterpart in the reference): the samples go over once, each step
+ * gathers its batch on the device, so training copies nothing per step. x, target, mask as above with n samples of
+ * h x w pixels. */
+int pcbprior_dataset(pcbprior* p, const float* x, const float* target, const float* mask, int n, int h, int w);
+/* one Adam step on samples idx[0..batch) of the resident set; loss may be NULL */
+int pcbprior_train_resident(pcbprior* p, const int32_t* idx, int batch, float lr, float* loss);
+/* the forward pass over samples idx[0..batch); logits (batch, h, w, out_ch) may be NULL */
+int pcbprior_forward_resident(pcbprior* p, const int32_t* idx, int batch, float* logits);
+
+/* Kernel accounting: while enabled, every conv3x3 launch (forward and input gradient) and weight-gradient launch
+ * is bracketed by HIP events on the network's stream; FLOPs count real channels (2 x pixels x cout x 9 x cin). */
+typedef struct pcbprior_stats {
+    uint64_t conv_launches, wgrad_launches;
+    double conv_ms, wgrad_ms;
+    double conv_flop, wgrad_flop;
+} pcbprior_stats;
+void pcbprior_stats_enable(pcbprior* p, int on);
+void pcbprior_stats_read(pcbprior* p, pcbprior_stats* s, int reset);
+
 /* One 16x16x16 BF16 matrix-core product, for the lane-layout test: a (16x16), b (16x16) row-major floats rounded
  * to BF16, d (16x16) = a @ b in FP32. */
 int pcbprior_mfma_check(const float* a, const float* b, float* d);

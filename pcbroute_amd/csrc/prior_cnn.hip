@@ -193,26 +193,58 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* x, const uint16_t
     if (threadIdx.x < C) atomicAdd(&db[threadIdx.x], bsum);
 }
 
-// binary cross-entropy of the logits over the masked pixels; dz = (sigmoid(z) - t) * m / count in BF16 slots
+// binary cross-entropy of the logits over the masked pixels; dz = (sigmoid(z) - t) m / count into the BF16 slots
+// (0 in the padded ones). A thread takes pixels in a grid-stride loop; one atomic per block adds its partial loss.
 __global__ __launch_bounds__(256) void k_loss(const float* z, const float* t, const float* m, uint16_t* dz, float* loss,
-                                              size_t npix, int cout, float inv) {
+                                              size_t npix, int cout, const unsigned* cnt) {
     __shared__ float part[4];
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    float g = 0.f, l = 0.f;
-    if (i < npix * C) {
-        const size_t p = i / C;
-        const int c = int(i % C);
-        if (c < cout && m[p] != 0.f) {
-            const float zz = z[p * cout + c], tt = t[p * cout + c];
-            g = (1.f / (1.f + expf(-zz)) - tt) * inv;
-            l = (fmaxf(zz, 0.f) - zz * tt + log1pf(expf(-fabsf(zz)))) * inv;
+    const float inv = *cnt ? 1.f / (float(*cnt) * float(cout)) : 0.f;
+    float l = 0.f;
+    for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < npix; p += size_t(gridDim.x) * blockDim.x) {
+        const bool on = m[p] != 0.f;
+        uint32_t w[C / 2];
+        for (int c = 0; c < C; c += 2) {
+            float g[2] = {0.f, 0.f};
+            for (int k = 0; k < 2; ++k)
+                if (on && c + k < cout) {
+                    const float zz = z[p * cout + c + k], tt = t[p * cout + c + k];
+                    g[k] = (1.f / (1.f + expf(-zz)) - tt) * inv;
+                    l += (fmaxf(zz, 0.f) - zz * tt + log1pf(expf(-fabsf(zz)))) * inv;
+                }
+            w[c / 2] = uint32_t(f2bf(g[0])) | (uint32_t(f2bf(g[1])) << 16);
         }
-        dz[i] = f2bf(g);
+        uint4* o = reinterpret_cast<uint4*>(dz + p * C);
+        for (int q = 0; q < C / 8; ++q) o[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
     }
     for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
     __syncthreads();
     if (threadIdx.x == 0) atomicAdd(loss, part[0] + part[1] + part[2] + part[3]);
+}
+
+// masked pixels of a batch (the loss's mean runs over them), one atomic per block
+__global__ __launch_bounds__(256) void k_count(const float* m, size_t npix, unsigned* cnt) {
+    __shared__ unsigned part[4];
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned c = (i < npix && m[i] != 0.f) ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(cnt, part[0] + part[1] + part[2] + part[3]);
+}
+
+// a batch from the resident training set: sample idx[b] into slot b (inputs in BF16 slots, targets, mask)
+__global__ void k_gather(const uint16_t* dsx, const float* dst, const float* dsm, const int32_t* idx, size_t spix,
+                         int cout, int batch, uint16_t* x, float* t, float* m) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;  // over batch * spix * C
+    if (i >= size_t(batch) * spix * C) return;
+    const size_t b = i / (spix * C), r = i % (spix * C), src = size_t(idx[b]) * spix * C + r;
+    x[i] = dsx[src];
+    if (r % C == 0) {
+        const size_t px = r / C, o = size_t(idx[b]) * spix + px, dpx = b * spix + px;
+        m[dpx] = dsm[o];
+        for (int c = 0; c < cout; ++c) t[dpx * cout + c] = dst[o * cout + c];
+    }
 }
 
 __global__ void k_sigmoid(const float* z, float* p, size_t n) {
@@ -281,8 +313,20 @@ struct pcbprior {
     std::vector<DB<uint16_t>> act;  // depth activations: act[0] the input, act[L] the input of layer L
     DB<uint16_t> dya, dyb;
     DB<float> logits, tgt, msk, xin, loss;
+    DB<unsigned> cnt;
     int step = 0;
     size_t npix = 0;
+    // the resident training set: inputs in BF16 slots, targets and masks, all in HBM
+    DB<uint16_t> dsx;
+    DB<float> dst, dsm;
+    DB<int32_t> didx;
+    int ds_n = 0, ds_h = 0, ds_w = 0;
+    // kernel accounting (events on the network's stream, read at the end of each call)
+    bool acct = false;
+    std::vector<hipEvent_t> evpool;
+    struct Pending { hipEvent_t a, b; int kind; double flop; };
+    std::vector<Pending> pend;
+    pcbprior_stats st{};
 };
 
 namespace {
@@ -315,17 +359,93 @@ bool refresh_bf16(pcbprior* p) {
     return true;
 }
 
+// kernel accounting: an event pair around a launch of kind 0 (conv: forward or input gradient) or 1 (weight
+// gradient) with its algorithmic FLOPs (real channels: 2 x pixels x cout x 9 x cin)
+hipEvent_t ev_take(pcbprior* p) {
+    if (!p->evpool.empty()) { hipEvent_t e = p->evpool.back(); p->evpool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+struct Timed {
+    pcbprior* p;
+    hipEvent_t a = nullptr;
+    int kind;
+    double flop;
+    Timed(pcbprior* p_, int kind_, double flop_) : p(p_), kind(kind_), flop(flop_) {
+        if (p->acct) { a = ev_take(p); (void)hipEventRecord(a, p->s); }
+    }
+    ~Timed() {
+        if (!p->acct) return;
+        hipEvent_t b = ev_take(p);
+        (void)hipEventRecord(b, p->s);
+        p->pend.push_back({a, b, kind, flop});
+    }
+};
+void collect(pcbprior* p) {  // after a stream sync
+    for (auto& e : p->pend) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e.a, e.b);
+        if (e.kind == 0) { ++p->st.conv_launches; p->st.conv_ms += ms; p->st.conv_flop += e.flop; }
+        else { ++p->st.wgrad_launches; p->st.wgrad_ms += ms; p->st.wgrad_flop += e.flop; }
+        p->evpool.push_back(e.a);
+        p->evpool.push_back(e.b);
+    }
+    p->pend.clear();
+}
+
+// the layers over act[0] (already on the device)
+bool forward_dev(pcbprior* p, int n, int h, int w) {
+    const dim3 grid(w / PX, h, n);
+    const double np = double(n) * h * w;
+    for (int L = 0; L < p->depth; ++L) {
+        const bool last = L == p->depth - 1;
+        Timed t(p, 0, 2.0 * np * p->lout[L] * 9 * p->lin[L]);
+        hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, p->s, p->act[L].p, p->wt.p + size_t(L) * C * KK,
+                           p->bias.p + L * C, last ? nullptr : p->act[L + 1].p, last ? p->logits.p : nullptr,
+                           static_cast<const uint16_t*>(nullptr), last ? 0 : 1, p->cout, h, w);
+    }
+    HIPOK(hipGetLastError());
+    return true;
+}
+
 bool forward(pcbprior* p, const float* x, int n, int h, int w) {
     if (!ensure(p, n, h, w)) return false;
     const size_t np = p->npix;
     HIPOK(hipMemcpyAsync(p->xin.p, x, np * p->cin * 4, hipMemcpyHostToDevice, p->s));
     hipLaunchKernelGGL(k_to_bf16, dim3(g1(np * C)), dim3(256), 0, p->s, p->xin.p, p->act[0].p, np, p->cin);
+    return forward_dev(p, n, h, w);
+}
+
+// loss and gradients; targets and mask already on the device
+bool backward_dev(pcbprior* p, int n, int h, int w) {
+    const size_t np = p->npix;
+    if (!p->cnt.need(1)) return false;
+    HIPOK(hipMemsetAsync(p->loss.p, 0, 4, p->s));
+    HIPOK(hipMemsetAsync(p->cnt.p, 0, 4, p->s));
+    HIPOK(hipMemsetAsync(p->g.p, 0, size_t(p->depth) * C * KK * 4, p->s));
+    HIPOK(hipMemsetAsync(p->gb.p, 0, size_t(p->depth) * C * 4, p->s));
+    hipLaunchKernelGGL(k_count, dim3(g1(np)), dim3(256), 0, p->s, p->msk.p, np, p->cnt.p);
+    hipLaunchKernelGGL(k_loss, dim3(unsigned(std::min<size_t>(g1(np), 2048))), dim3(256), 0, p->s, p->logits.p, p->tgt.p,
+                       p->msk.p, p->dya.p, p->loss.p, np, p->cout, static_cast<const unsigned*>(p->cnt.p));
     const dim3 grid(w / PX, h, n);
-    for (int L = 0; L < p->depth; ++L) {
-        const bool last = L == p->depth - 1;
-        hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, p->s, p->act[L].p, p->wt.p + size_t(L) * C * KK,
-                           p->bias.p + L * C, last ? nullptr : p->act[L + 1].p, last ? p->logits.p : nullptr,
-                           static_cast<const uint16_t*>(nullptr), last ? 0 : 1, p->cout, h, w);
+    uint16_t* dy = p->dya.p;
+    uint16_t* dx = p->dyb.p;
+    for (int L = p->depth - 1; L >= 0; --L) {
+        const double fl = 2.0 * double(np) * p->lout[L] * 9 * p->lin[L];
+        {
+            Timed t(p, 1, fl);
+            hipLaunchKernelGGL(k_wgrad, dim3(w / PX, (h + WG_ROWS - 1) / WG_ROWS, n), dim3(256), 0, p->s, p->act[L].p,
+                               dy, p->g.p + size_t(L) * C * KK, p->gb.p + L * C, h, w);
+        }
+        if (L > 0) {
+            // the gradient at layer L's input: the flipped-weight conv of dy, zero where that input's ReLU was off
+            Timed t(p, 0, fl);
+            hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, p->s, dy, p->wf.p + size_t(L) * C * KK,
+                               static_cast<const float*>(nullptr), dx, static_cast<float*>(nullptr), p->act[L].p, 0, 0,
+                               h, w);
+            std::swap(dy, dx);
+        }
     }
     HIPOK(hipGetLastError());
     return true;
@@ -335,35 +455,40 @@ bool backward(pcbprior* p, const float* target, const float* mask, int n, int h,
     const size_t np = p->npix;
     HIPOK(hipMemcpyAsync(p->tgt.p, target, np * p->cout * 4, hipMemcpyHostToDevice, p->s));
     HIPOK(hipMemcpyAsync(p->msk.p, mask, np * 4, hipMemcpyHostToDevice, p->s));
-    std::vector<float> mh(mask, mask + np);
-    double cnt = 0;
-    for (float v : mh) cnt += v != 0.f;
-    const float inv = cnt > 0 ? float(1.0 / (cnt * p->cout)) : 0.f;
-    HIPOK(hipMemsetAsync(p->loss.p, 0, 4, p->s));
-    HIPOK(hipMemsetAsync(p->g.p, 0, size_t(p->depth) * C * KK * 4, p->s));
-    HIPOK(hipMemsetAsync(p->gb.p, 0, size_t(p->depth) * C * 4, p->s));
-    hipLaunchKernelGGL(k_loss, dim3(g1(np * C)), dim3(256), 0, p->s, p->logits.p, p->tgt.p, p->msk.p, p->dya.p,
-                       p->loss.p, np, p->cout, inv);
-    const dim3 grid(w / PX, h, n);
-    uint16_t* dy = p->dya.p;
-    uint16_t* dx = p->dyb.p;
-    for (int L = p->depth - 1; L >= 0; --L) {
-        hipLaunchKernelGGL(k_wgrad, dim3(w / PX, (h + WG_ROWS - 1) / WG_ROWS, n), dim3(256), 0, p->s, p->act[L].p, dy,
-                           p->g.p + size_t(L) * C * KK,
-                           p->gb.p + L * C, h, w);
-        if (L > 0) {
-            // the gradient at layer L's input: the flipped-weight conv of dy, zero where that input's ReLU was off
-            hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, p->s, dy, p->wf.p + size_t(L) * C * KK,
-                               static_cast<const float*>(nullptr), dx, static_cast<float*>(nullptr), p->act[L].p, 0, 0,
-                               h, w);
-            std::swap(dy, dx);
-        }
-    }
-    HIPOK(hipGetLastError());
+    if (!backward_dev(p, n, h, w)) return false;
     if (loss_out) {
         HIPOK(hipMemcpyAsync(loss_out, p->loss.p, 4, hipMemcpyDeviceToHost, p->s));
         HIPOK(hipStreamSynchronize(p->s));
+        collect(p);
     }
+    return true;
+}
+
+// one Adam step on the gradients just computed (FP32 masters; BF16 copies refreshed)
+bool adam(pcbprior* p, float lr) {
+    ++p->step;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - std::pow(b1, float(p->step)), c2 = 1.f - std::pow(b2, float(p->step));
+    const size_t nw = size_t(p->depth) * C * KK, nb = size_t(p->depth) * C;
+    hipLaunchKernelGGL(k_adam, dim3(g1(nw)), dim3(256), 0, p->s, p->w.p, p->g.p, p->mo.p, p->ve.p, nw, lr, b1, b2,
+                       eps, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3(g1(nb)), dim3(256), 0, p->s, p->bias.p, p->gb.p, p->mb.p, p->vb.p, nb, lr, b1, b2,
+                       eps, c1, c2);
+    return refresh_bf16(p);
+}
+
+// batch idx[0..batch) of the resident set into act[0], tgt and msk
+bool gather(pcbprior* p, const int32_t* idx, int batch) {
+    if (!p->ds_n) return fail("no resident training set");
+    if (batch < 1) return fail("empty batch");
+    for (int b = 0; b < batch; ++b)
+        if (idx[b] < 0 || idx[b] >= p->ds_n) return fail("sample index out of range");
+    if (!ensure(p, batch, p->ds_h, p->ds_w) || !p->didx.need(size_t(batch)) || !h2d(p, p->didx.p, idx, size_t(batch) * 4))
+        return false;
+    const size_t spix = size_t(p->ds_h) * p->ds_w;
+    hipLaunchKernelGGL(k_gather, dim3(g1(size_t(batch) * spix * C)), dim3(256), 0, p->s, p->dsx.p, p->dst.p, p->dsm.p,
+                       p->didx.p, spix, p->cout, batch, p->act[0].p, p->tgt.p, p->msk.p);
+    HIPOK(hipGetLastError());
     return true;
 }
 
@@ -424,6 +549,9 @@ pcbprior* pcbprior_create(int in_ch, int width, int depth, int out_ch, uint64_t 
 
 void pcbprior_destroy(pcbprior* p) {
     if (!p) return;
+    if (p->s) (void)hipStreamSynchronize(p->s);
+    collect(p);
+    for (auto e : p->evpool) (void)hipEventDestroy(e);
     if (p->s) (void)hipStreamDestroy(p->s);
     delete p;
 }
@@ -477,6 +605,7 @@ int pcbprior_logits(pcbprior* p, const float* x, int n, int h, int w, float* log
     if (hipMemcpyAsync(logits, p->logits.p, p->npix * p->cout * 4, hipMemcpyDeviceToHost, p->s) != hipSuccess ||
         hipStreamSynchronize(p->s) != hipSuccess)
         return -1;
+    collect(p);
     return 0;
 }
 
@@ -489,6 +618,7 @@ int pcbprior_predict(pcbprior* p, const float* x, int n, int h, int w, float* pr
     if (hipMemcpyAsync(prob, p->logits.p, nz * 4, hipMemcpyDeviceToHost, p->s) != hipSuccess ||
         hipStreamSynchronize(p->s) != hipSuccess)
         return -1;
+    collect(p);
     return 0;
 }
 
@@ -517,19 +647,60 @@ int pcbprior_train_step(pcbprior* p, const float* x, const float* target, const 
                         float lr, float* loss) {
     g_err.clear();
     if (!p || !x || !target || !mask) return -1;
-    if (!forward(p, x, n, h, w) || !backward(p, target, mask, n, h, w, nullptr)) return -1;
-    ++p->step;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const float c1 = 1.f - std::pow(b1, float(p->step)), c2 = 1.f - std::pow(b2, float(p->step));
-    const size_t nw = size_t(p->depth) * C * KK, nb = size_t(p->depth) * C;
-    hipLaunchKernelGGL(k_adam, dim3(g1(nw)), dim3(256), 0, p->s, p->w.p, p->g.p, p->mo.p, p->ve.p, nw, lr, b1, b2,
-                       eps, c1, c2);
-    hipLaunchKernelGGL(k_adam, dim3(g1(nb)), dim3(256), 0, p->s, p->bias.p, p->gb.p, p->mb.p, p->vb.p, nb, lr, b1, b2,
-                       eps, c1, c2);
-    if (!refresh_bf16(p)) return -1;
-    if (loss && (hipMemcpyAsync(loss, p->loss.p, 4, hipMemcpyDeviceToHost, p->s) != hipSuccess)) return -1;
-    if (hipStreamSynchronize(p->s) != hipSuccess) return -1;
+    if (!forward(p, x, n, h, w) || !backward(p, target, mask, n, h, w, nullptr) || !adam(p, lr)) return -1;
+    if (loss && !d2h(p, loss, p->loss.p, 4)) return -1;
+    if (!sync(p)) return -1;
+    collect(p);
     return 0;
+}
+
+int pcbprior_dataset(pcbprior* p, const float* x, const float* target, const float* mask, int n, int h, int w) {
+    g_err.clear();
+    if (!p || !x || !target || !mask || n < 1 || h < 1 || w < 1 || w % PX) { g_err = "bad training set"; return -1; }
+    const size_t spix = size_t(h) * w, tot = size_t(n) * spix;
+    if (!p->dsx.need(tot * C) || !p->dst.need(tot * p->cout) || !p->dsm.need(tot) || !p->xin.need(spix * p->cin)) return -1;
+    // inputs go over a sample at a time and become BF16 slots on the device
+    for (int k = 0; k < n; ++k) {
+        if (!h2d(p, p->xin.p, x + size_t(k) * spix * p->cin, spix * p->cin * 4)) return -1;
+        hipLaunchKernelGGL(k_to_bf16, dim3(g1(spix * C)), dim3(256), 0, p->s, p->xin.p, p->dsx.p + size_t(k) * spix * C,
+                           spix, p->cin);
+        if (!sync(p)) return -1;  // xin is reused by the next sample
+    }
+    if (!h2d(p, p->dst.p, target, tot * p->cout * 4) || !h2d(p, p->dsm.p, mask, tot * 4) || !sync(p)) return -1;
+    p->ds_n = n; p->ds_h = h; p->ds_w = w;
+    return 0;
+}
+
+int pcbprior_train_resident(pcbprior* p, const int32_t* idx, int batch, float lr, float* loss) {
+    g_err.clear();
+    if (!p || !idx) return -1;
+    if (!gather(p, idx, batch) || !forward_dev(p, batch, p->ds_h, p->ds_w) || !backward_dev(p, batch, p->ds_h, p->ds_w) ||
+        !adam(p, lr))
+        return -1;
+    if (loss && !d2h(p, loss, p->loss.p, 4)) return -1;
+    if (!sync(p)) return -1;
+    collect(p);
+    return 0;
+}
+
+int pcbprior_forward_resident(pcbprior* p, const int32_t* idx, int batch, float* logits) {
+    g_err.clear();
+    if (!p || !idx) return -1;
+    if (!gather(p, idx, batch) || !forward_dev(p, batch, p->ds_h, p->ds_w)) return -1;
+    if (logits && !d2h(p, logits, p->logits.p, p->npix * p->cout * 4)) return -1;
+    if (!sync(p)) return -1;
+    collect(p);
+    return 0;
+}
+
+void pcbprior_stats_enable(pcbprior* p, int on) {
+    if (p) p->acct = on != 0;
+}
+
+void pcbprior_stats_read(pcbprior* p, pcbprior_stats* st, int reset) {
+    if (!p) return;
+    if (st) *st = p->st;
+    if (reset) p->st = pcbprior_stats{};
 }
 
 int pcbprior_mfma_check(const float* a, const float* b, float* d) {

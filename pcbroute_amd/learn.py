@@ -53,16 +53,15 @@ def collect(seed, wave=0):
 
 
 def train(net, data, epochs=4, batch=16, lr=1e-3, seed=0):
+    """Adam over shuffled batches; the training set stays resident in HBM (one upload)."""
     x, t, m = data
+    net.dataset(x, t, m)
     rng = np.random.default_rng(seed)
     n = len(x)
     losses = []
     for ep in range(epochs):
-        order = rng.permutation(n)
-        el = []
-        for k in range(0, n - batch + 1, batch):
-            idx = order[k:k + batch]
-            el.append(net.train_step(x[idx], t[idx], m[idx], lr=lr))
+        order = rng.permutation(n).astype(np.int32)
+        el = [net.train_resident(order[k:k + batch], lr=lr) for k in range(0, n - batch + 1, batch)]
         losses.append(float(np.mean(el)))
         log(f"epoch {ep + 1}/{epochs}: loss {losses[-1]:.4f}")
     return losses

@@ -15,6 +15,11 @@ MAX_CH = 32
 _F = C.POINTER(C.c_float)
 
 
+class Stats(C.Structure):
+    _fields_ = [("conv_launches", C.c_uint64), ("wgrad_launches", C.c_uint64), ("conv_ms", C.c_double),
+                ("wgrad_ms", C.c_double), ("conv_flop", C.c_double), ("wgrad_flop", C.c_double)]
+
+
 def _bind(lib):
     lib.pcbprior_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]
     lib.pcbprior_create.restype = C.c_void_p
@@ -28,6 +33,11 @@ def _bind(lib):
     lib.pcbprior_loss_grad.argtypes = [C.c_void_p, _F, _F, _F, C.c_int, C.c_int, C.c_int, _F, _F]
     lib.pcbprior_train_step.argtypes = [C.c_void_p, _F, _F, _F, C.c_int, C.c_int, C.c_int, C.c_float, _F]
     lib.pcbprior_mfma_check.argtypes = [_F, _F, _F]
+    lib.pcbprior_dataset.argtypes = [C.c_void_p, _F, _F, _F, C.c_int, C.c_int, C.c_int]
+    lib.pcbprior_train_resident.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int, C.c_float, _F]
+    lib.pcbprior_forward_resident.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int, _F]
+    lib.pcbprior_stats_enable.argtypes = [C.c_void_p, C.c_int]
+    lib.pcbprior_stats_read.argtypes = [C.c_void_p, C.POINTER(Stats), C.c_int]
     lib.pcbprior_last_error.restype = C.c_char_p
     return lib
 
@@ -150,6 +160,35 @@ class PriorNet:
         g = np.zeros(self.num_params, np.float32)
         self._check(self.lib.pcbprior_loss_grad(self.h, _p(x), _p(t), _p(m), n, h, w, _p(loss), _p(g)))
         return float(loss[0]), g
+
+    def dataset(self, x, target, mask):
+        """Makes (x, target, mask) the resident training set (kept in HBM as BF16 inputs)."""
+        x, t, m, n, h, w = self._tm(x, target, mask)
+        self._check(self.lib.pcbprior_dataset(self.h, _p(x), _p(t), _p(m), n, h, w))
+        self._ds = (n, h, w)
+
+    def train_resident(self, idx, lr=1e-3, loss=True):
+        """One Adam step on samples idx of the resident set; the loss (or None)."""
+        idx = np.ascontiguousarray(idx, np.int32)
+        out = np.zeros(1, np.float32)
+        self._check(self.lib.pcbprior_train_resident(self.h, idx.ctypes.data_as(C.POINTER(C.c_int32)), len(idx), lr,
+                                                     _p(out) if loss else None))
+        return float(out[0]) if loss else None
+
+    def forward_resident(self, idx, logits=True):
+        idx = np.ascontiguousarray(idx, np.int32)
+        n, h, w = self._ds
+        out = np.zeros((len(idx), h, w, self.out_ch), np.float32) if logits else None
+        self._check(self.lib.pcbprior_forward_resident(self.h, idx.ctypes.data_as(C.POINTER(C.c_int32)), len(idx),
+                                                       _p(out) if logits else None))
+        return out
+
+    def stats(self, enable=None, reset=False):
+        if enable is not None:
+            self.lib.pcbprior_stats_enable(self.h, int(bool(enable)))
+        st = Stats()
+        self.lib.pcbprior_stats_read(self.h, C.byref(st), int(reset))
+        return {k: getattr(st, k) for k, _ in Stats._fields_}
 
     def train_step(self, x, target, mask, lr=1e-3):
         x, t, m, n, h, w = self._tm(x, target, mask)

@@ -137,3 +137,40 @@ def test_bad_shapes_raise(prior):
     with pytest.raises(prior.BackendError):
         prior.PriorNet(40, 8, 2, 1)  # more channels than a layer carries
     net.close()
+
+
+def test_resident_training_matches_host_batches(prior):
+    """A step on the resident set equals the step on the same batch passed from the host; forward_resident equals
+    logits; the accounting counts every conv launch with the algorithmic FLOPs."""
+    rng = np.random.default_rng(21)
+    cin, width, depth, cout = 6, 16, 3, 2
+    params = R.flatten(R.random_params(rng, cin, width, depth, cout))
+    x = rng.standard_normal((5, 8, 64, cin)).astype(np.float32)
+    t = (rng.random((5, 8, 64, cout)) < 0.3).astype(np.float32)
+    m = (rng.random((5, 8, 64)) < 0.9).astype(np.float32)
+    a = prior.PriorNet(cin, width, depth, cout)
+    b = prior.PriorNet(cin, width, depth, cout)
+    a.set_params(params)
+    b.set_params(params)
+    idx = np.array([3, 0, 4], np.int32)
+    b.dataset(x, t, m)
+    np.testing.assert_allclose(b.forward_resident(idx), a.logits(x[idx]), rtol=1e-6, atol=1e-6)
+    b.stats(enable=True, reset=True)
+    la = a.train_step(x[idx], t[idx], m[idx], lr=1e-2)
+    lb = b.train_resident(idx, lr=1e-2)
+    st = b.stats(enable=False, reset=True)
+    assert abs(la - lb) <= 1e-6 * (1 + abs(la))
+    # the weight gradients sum FP32 partials with atomics, so the two steps differ in summation order only; Adam turns
+    # a near-zero gradient into a full step of either sign, so a few parameters may differ by up to 2 lr
+    pa, pb = a.get_params(), b.get_params()
+    off = np.abs(pa - pb) > 1e-5 * (1 + np.abs(pa))
+    assert off.mean() < 0.01 and np.abs(pa - pb).max() <= 2.01e-2
+    assert st["conv_launches"] == depth + depth - 1 and st["wgrad_launches"] == depth
+    npx = 3 * 8 * 64
+    fwd = 2 * npx * 9 * (cin * width + width * width + width * cout)
+    assert abs(st["conv_flop"] - fwd - (fwd - 2 * npx * 9 * cin * width)) < 1 and abs(st["wgrad_flop"] - fwd) < 1
+    assert st["conv_ms"] > 0 and st["wgrad_ms"] > 0
+    with pytest.raises(prior.BackendError):
+        b.train_resident(np.array([5], np.int32))  # out of range
+    a.close()
+    b.close()
