@@ -8,8 +8,9 @@ the same searches one after another (one solve at a time) are timed beside it. P
 value: grid states solved per second (64 x 12.8M states per step / wall time of the round mode).
 roofline: the sweep kernel k_pass, algorithmic bytes = 12 B per state carried through a tile visit (int32 cost word
 read, int32 field read and write) / summed kernel time from HIP events on the launch stream.
-Tile rows, local sweeps and the delta band come from the library's autotuner, run before the timed region on a
-sample of the same workload drawn with another seed (--tune off: the library defaults or the explicit flags).
+Tile rows, local sweeps and the delta band: the library's defaults for this workload (16-row tiles, up to 4 local
+sweeps, automatic band), so the committed rocprof traffic profile matches the run; --tune auto runs the autotuner on
+a sample drawn with another seed first. The batched round workload tunes by default.
 
 --gpus N with N > 1 exits: the back end solves each search on a single device.
 """
@@ -223,16 +224,17 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--local-sweeps", type=int, default=0)
     ap.add_argument("--delta", type=int, default=DEFAULT_DELTA, help="delta-stepping band over tiles (0: automatic, -1: off)")
-    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=("both", "single", "round"), default="both")
     ap.add_argument("--workload", choices=("kernel", "round", "prior"), default="kernel",
                     help="kernel: the 64 whole-grid searches (the metric); round: 256 net searches of a speculative "
                          "round on a 2400x1800x6 board with two net classes, as one round and one at a time; "
                          "prior: the corridor CNN's training and inference on full 4-layer boards")
-    ap.add_argument("--tune", choices=("auto", "off"), default="auto",
+    ap.add_argument("--tune", choices=("auto", "off"), default=None,
                     help="auto: the autotuner picks tile rows, local sweeps and delta on a sample drawn with another "
-                         "seed (the default unless --tile-rows, --local-sweeps or --delta is given)")
+                         "seed (the default for --workload round); off: library defaults or the explicit flags (the "
+                         "default for the kernel workload)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 4 times (calibrates the HBM byte counters) and exit")
     a = ap.parse_args()
@@ -255,6 +257,8 @@ def main():
     b = Backend()
     if not b.available():
         raise SystemExit("no HIP device")
+    if a.tune is None:
+        a.tune = "auto" if a.workload == "round" else "off"
     if a.tile_rows or a.local_sweeps or a.delta != DEFAULT_DELTA:
         a.tune = "off"
     if a.tile_rows or a.local_sweeps:
