@@ -72,6 +72,29 @@ def run(b, g, probs, mode, steps, warmup):
     return wall, st, sweeps
 
 
+def round_with_fields(b, g, probs):
+    """Wall time of one round whose canonical fields come back into the library's pinned output buffers (the
+    pcbgpu_round_* calls directly: no copy into Python arrays)."""
+    import ctypes as C
+    from pcbroute_amd.abi import Out
+    lib = b.lib
+    qs = [p.query() for p in probs]
+    r = lib.pcbgpu_round_begin()
+    for q in qs:
+        lib.pcbgpu_round_submit(r, g.h, C.byref(q))
+    t0 = time.perf_counter()
+    rc = lib.pcbgpu_round_launch(r)
+    dt = time.perf_counter() - t0
+    for k in range(len(qs)):
+        o = Out()
+        lib.pcbgpu_round_result(r, k, C.byref(o))
+        lib.pcbgpu_out_release(C.byref(o))
+    lib.pcbgpu_round_end(r)
+    if rc != 0:
+        raise RuntimeError(b.error())
+    return dt
+
+
 def tune(b, pairs, a):
     """The autotuner on a sample round (another seed than the timed searches); the chosen setting, or None."""
     if a.tune == "off":
@@ -327,6 +350,13 @@ def main():
         if t.get("config") == want:
             line["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
             line["roofline"]["traffic_source"] = t["source"]
+    # the same round with every search's canonical field copied back into pinned host memory (what a caller that
+    # backtraces on the host pays): reported beside the value, never as it
+    g2 = b.grid(probs[0])
+    t_fields = round_with_fields(b, g2, probs)
+    line["with_fields"] = {"ms_per_search": round(t_fields / len(probs) * 1e3, 3),
+                           "field_mb_per_search": round(cells * 4 / 1e6, 1)}
+    g2.close()
     if "single" in res and head != "single":
         w1, s1, sw1 = res["single"]
         line["one_at_a_time"] = {"ms_per_search": round(w1 / n_search * 1e3, 3),
