@@ -123,6 +123,9 @@ pcbgpu_grid* pcbgpu_grid_create(int32_t nx, int32_t ny, int32_t nl);
 void pcbgpu_grid_destroy(pcbgpu_grid* g);
 /* copy rows [j0, j0 + nrows) of layer l (l == nl: the via grid) from host arrays laid out like the query's */
 int pcbgpu_grid_upload(pcbgpu_grid* g, int32_t l, int32_t j0, int32_t nrows, const int32_t* cst, const uint16_t* st);
+/* the same for the cells [i0, i0 + w) x [j0, j0 + h) only (a commit's footprint) */
+int pcbgpu_grid_upload_rect(pcbgpu_grid* g, int32_t l, int32_t i0, int32_t j0, int32_t w, int32_t h, const int32_t* cst,
+                            const uint16_t* st);
 /* a solve whose cst/st/vcst/vst come from the grid (the query's pointers for them are ignored) */
 int pcbgpu_solve_grid(const pcbgpu_grid* g, const pcbgpu_query* q, pcbgpu_out* out, int wait);
 /* one rectangle of the window, no widening (src/gpu.cu:422-546 solve_rect). flags: PCBGPU_NOFIELD leaves the

@@ -65,6 +65,20 @@ int pcbgpu_grid_upload(pcbgpu_grid* g, int32_t l, int32_t j0, int32_t nrows, con
     return 0;
 }
 
+int pcbgpu_grid_upload_rect(pcbgpu_grid* g, int32_t l, int32_t i0, int32_t j0, int32_t w, int32_t h, const int32_t* cst,
+                            const uint16_t* st) {
+    if (!g || l < 0 || l > g->nl || i0 < 0 || j0 < 0 || w < 0 || h < 0 || i0 + w > g->nx || j0 + h > g->ny) return -1;
+    const size_t n = size_t(g->nx) * g->ny;
+    int32_t* dc = l < g->nl ? g->cst.data() + size_t(l) * n : g->vcst.data();
+    uint16_t* ds = l < g->nl ? g->st.data() + size_t(l) * n : g->vst.data();
+    for (int j = j0; j < j0 + h; ++j) {
+        const size_t off = size_t(j) * g->nx + i0;
+        std::memcpy(dc + off, cst + off, size_t(w) * 4);
+        std::memcpy(ds + off, st + off, size_t(w) * 2);
+    }
+    return 0;
+}
+
 int pcbgpu_solve_grid(const pcbgpu_grid* g, const pcbgpu_query* q, pcbgpu_out* out, int) {
     const pcbgpu_query x = on_grid(g, q);
     return orc_solve(&x, out, size_t(1) << 31);

@@ -78,40 +78,101 @@ __global__ void k_to_bf16(const float* x, uint16_t* out, size_t npix, int cin) {
     out[i] = c < cin ? f2bf(x[p * cin + c]) : uint16_t(0);
 }
 
-// y = conv3x3(x, wt) + bias: the implicit GEMM above. Outputs BF16 (32 slots, ReLU optional, zeroed where the mask
-// activation is zero: the derivative of the layer before's ReLU) or FP32 logits of the first cout channels.
+// y = conv3x3(x, wt) + bias, the implicit GEMM above. A block takes 64 pixels of CR consecutive rows, one row per
+// wavefront, all 32 output channels. It stages the (CR + 2) x 66-pixel input patch and the layer's weights in LDS
+// once; a wavefront then holds 4 x 2 accumulator tiles (64 pixels x 32 channels) and per K step reads 4 A and 2 B
+// fragments from LDS for 8 matrix-core products. The results go through LDS so that each lane finishes one pixel:
+// bias, ReLU, the mask (zero where the mask activation is zero: the derivative of the layer before's ReLU), then
+// coalesced BF16 stores of its 32 slots, or FP32 logits of the first cout channels.
+constexpr int CR = 4;         // rows per block
+constexpr int XSP = 36;       // LDS pitch of a staged input pixel (BF16): 18 dwords, conflict-free 8-byte reads
+constexpr int WSP = KK + 8;   // LDS pitch of a staged weight row
+constexpr int OSP = C + 1;    // LDS pitch of an output pixel (FP32)
+constexpr int XS_N = (CR + 2) * (PX + 2) * XSP, WS_N = C * WSP;  // BF16 elements
+static_assert(CR * PX * OSP * 2 <= XS_N + WS_N, "the output tile reuses the staging space");
 __global__ __launch_bounds__(256) void k_conv(const uint16_t* x, const uint16_t* wt, const float* bias, uint16_t* ybf,
                                               float* yf, const uint16_t* mask, int relu, int cout, int H, int W) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int n = blockIdx.z, y = blockIdx.y, x0 = blockIdx.x * PX + wave * 16;
-    const int pr = lane & 15, kq = lane >> 4;  // A row (pixel) / B column (channel), k quarter
-    v4f acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+    __shared__ __align__(16) uint16_t sm[XS_N + WS_N];
+    uint16_t* XS = sm;
+    uint16_t* WS = sm + XS_N;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = blockIdx.z, y0 = blockIdx.y * CR, x0 = blockIdx.x * PX;
     const size_t img = size_t(n) * H * W;
-    for (int kk = 0; kk < KK / 16; ++kk) {
-        const int d = kk >> 1, half = kk & 1, dy = d / 3 - 1, dx = d % 3 - 1;
-        const int py = y + dy, px = x0 + pr + dx, ci = half * 16 + kq * 4;
-        v4s a = {0, 0, 0, 0};
-        if (py >= 0 && py < H && px >= 0 && px < W) a = load4(x + (img + size_t(py) * W + px) * C + ci);
-        const int k0 = kk * 16 + kq * 4;
-        const v4s b0 = load4(wt + size_t(pr) * KK + k0);
-        const v4s b1 = load4(wt + size_t(16 + pr) * KK + k0);
-        acc0 = mfma16(a, b0, acc0);
-        acc1 = mfma16(a, b1, acc1);
+    for (int e = tid; e < C * KK / 8; e += 256) {  // weights: 8 BF16 per load
+        const int co = e / (KK / 8), q = e % (KK / 8);
+        const uint4 v = *reinterpret_cast<const uint4*>(wt + size_t(co) * KK + q * 8);
+        *reinterpret_cast<uint4*>(WS + co * WSP + q * 8) = v;
     }
+    for (int e = tid; e < (CR + 2) * (PX + 2) * (C / 8); e += 256) {  // the input patch, zero outside the image
+        const int r = e / ((PX + 2) * (C / 8)), rem = e % ((PX + 2) * (C / 8)), q = rem / (C / 8), c8 = rem % (C / 8);
+        const int py = y0 - 1 + r, px = x0 - 1 + q;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (py >= 0 && py < H && px >= 0 && px < W) v = *reinterpret_cast<const uint4*>(x + (img + size_t(py) * W + px) * C + c8 * 8);
+        uint2* d = reinterpret_cast<uint2*>(XS + (r * (PX + 2) + q) * XSP + c8 * 8);
+        d[0] = make_uint2(v.x, v.y);
+        d[1] = make_uint2(v.z, v.w);
+    }
+    __syncthreads();
+    const int pr = lane & 15, kq = lane >> 4;
+    v4f acc[4][2];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const int co = 16 * t + pr;
-        const float bv = bias ? bias[co] : 0.f;
+    for (int m = 0; m < 4; ++m) acc[m][0] = acc[m][1] = v4f{0, 0, 0, 0};
+    for (int kk = 0; kk < KK / 16; ++kk) {
+        const int d = kk >> 1, dy = d / 3, dx = d % 3, ci = (kk & 1) * 16 + kq * 4;
+        const v4s b0 = load4(WS + pr * WSP + kk * 16 + kq * 4);
+        const v4s b1 = load4(WS + (16 + pr) * WSP + kk * 16 + kq * 4);
+        const uint16_t* xr = XS + ((wave + dy) * (PX + 2) + pr + dx) * XSP + ci;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int px = x0 + kq * 4 + j;
-            if (px >= W) continue;
-            const size_t p = img + size_t(y) * W + px;
-            float v = (t ? acc1[j] : acc0[j]) + bv;
-            if (relu) v = fmaxf(v, 0.f);
-            if (mask && (mask[p * C + co] & 0x7FFF) == 0) v = 0.f;  // the ReLU was off there
-            if (ybf) ybf[p * C + co] = f2bf(v);
-            if (yf && co < cout) yf[p * cout + co] = v;
+        for (int m = 0; m < 4; ++m) {
+            const v4s a = load4(xr + m * 16 * XSP);
+            acc[m][0] = mfma16(a, b0, acc[m][0]);
+            acc[m][1] = mfma16(a, b1, acc[m][1]);
+        }
+    }
+    __syncthreads();  // the staging space becomes the output tile
+    float* O = reinterpret_cast<float*>(sm) + wave * PX * OSP;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) O[(m * 16 + kq * 4 + j) * OSP + 16 * t + pr] = acc[m][t][j];
+    __syncthreads();
+    const int y = y0 + wave;
+    if (y >= H) return;
+    const size_t p = img + size_t(y) * W + x0 + lane;  // W is a multiple of 64: every lane's pixel is inside
+    const float* o = O + lane * OSP;
+    if (yf) {
+        for (int co = 0; co < cout; ++co) {
+            float v = o[co] + (bias ? bias[co] : 0.f);
+            yf[p * cout + co] = relu ? fmaxf(v, 0.f) : v;
+        }
+    }
+    if (ybf) {
+        uint4 mk[C / 8];
+        if (mask)
+#pragma unroll
+            for (int q = 0; q < C / 8; ++q) mk[q] = reinterpret_cast<const uint4*>(mask + p * C)[q];
+#pragma unroll
+        for (int q = 0; q < C / 8; ++q) {
+            uint32_t w[4];
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                uint32_t pair = 0;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int co = q * 8 + h * 2 + k;
+                    float v = o[co] + (bias ? bias[co] : 0.f);
+                    if (relu) v = fmaxf(v, 0.f);
+                    if (mask) {
+                        const uint32_t mw = (&mk[q].x)[h];
+                        if (((mw >> (16 * k)) & 0x7FFF) == 0) v = 0.f;  // the ReLU was off there
+                    }
+                    pair |= uint32_t(f2bf(v)) << (16 * k);
+                }
+                w[h] = pair;
+            }
+            reinterpret_cast<uint4*>(ybf + p * C)[q] = make_uint4(w[0], w[1], w[2], w[3]);
         }
     }
 }
@@ -396,7 +457,7 @@ void collect(pcbprior* p) {  // after a stream sync
 
 // the layers over act[0] (already on the device)
 bool forward_dev(pcbprior* p, int n, int h, int w) {
-    const dim3 grid(w / PX, h, n);
+    const dim3 grid(w / PX, (h + CR - 1) / CR, n);
     const double np = double(n) * h * w;
     for (int L = 0; L < p->depth; ++L) {
         const bool last = L == p->depth - 1;
@@ -428,7 +489,7 @@ bool backward_dev(pcbprior* p, int n, int h, int w) {
     hipLaunchKernelGGL(k_count, dim3(g1(np)), dim3(256), 0, p->s, p->msk.p, np, p->cnt.p);
     hipLaunchKernelGGL(k_loss, dim3(unsigned(std::min<size_t>(g1(np), 2048))), dim3(256), 0, p->s, p->logits.p, p->tgt.p,
                        p->msk.p, p->dya.p, p->loss.p, np, p->cout, static_cast<const unsigned*>(p->cnt.p));
-    const dim3 grid(w / PX, h, n);
+    const dim3 grid(w / PX, (h + CR - 1) / CR, n);
     uint16_t* dy = p->dya.p;
     uint16_t* dx = p->dyb.p;
     for (int L = p->depth - 1; L >= 0; --L) {

@@ -84,9 +84,9 @@ def corridor_stats(net, data, thr=0.05):
 
 CONFIGS = {  # name: (wave, group_mm, prior)
     "plain": (0, 1.0, False),
-    "batched": (-64, 4.0, False),
+    "batched": (16, 1.0, False),
     "learned": (0, 1.0, True),
-    "batched_learned": (-64, 4.0, True),
+    "batched_learned": (16, 1.0, True),
 }
 FIELDS = ("opens", "violations", "route_s", "search_s", "prior_s", "searches", "rounds", "revalidated", "iterations",
           "search_fallbacks", "length_cells", "vias")
