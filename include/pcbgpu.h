@@ -158,8 +158,8 @@ void pcbgpu_round_end(pcbgpu_round* r);
 int pcbgpu_set_tile(int32_t tile_rows, int32_t local_sweeps);
 void pcbgpu_get_tile(int32_t nl, int32_t* tile_rows, int32_t* local_sweeps);
 /* delta-stepping band over tiles: a pass visits the queued tiles whose key (the least cost change that queued them)
- * lies within delta of the least key; 0 picks 80 times the cheapest straight move weight (the default), -1 visits
- * every queued tile. Results do not depend on it. 0 or -1. */
+ * lies within delta of the least key; 0 (the default) picks 80 times the cheapest straight move weight, or no band
+ * for rectangles of at most 512 tiles; -1 visits every queued tile. Results do not depend on it. 0 or -1. */
 int pcbgpu_set_delta(int32_t delta);
 /* ---- autotuner (no counterpart in the reference). Tile rows per wavefront, local sweeps per tile visit and the
  * delta band, picked per layer count by timing a sample round; a picked setting takes precedence over

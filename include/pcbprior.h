@@ -35,6 +35,10 @@ int pcbprior_set_params(pcbprior* p, const float* in);
 
 /* Forward pass: per-pixel corridor probabilities (sigmoid of the logits), out (n, h, w, out_ch). */
 int pcbprior_predict(pcbprior* p, const float* x, int n, int h, int w, float* prob);
+/* The same over a batch of images of different sizes padded to h x w: image k's valid extent is hw[2k] rows by
+ * hw[2k + 1] columns (a multiple of 64), its inputs zero outside it. Every output inside an image's extent equals
+ * pcbprior_predict on that image alone at its extent; outputs outside are sigmoid(0). hw NULL: all valid. */
+int pcbprior_predict_valid(pcbprior* p, const float* x, int n, int h, int w, const int32_t* hw, float* prob);
 /* Logits instead of probabilities. */
 int pcbprior_logits(pcbprior* p, const float* x, int n, int h, int w, float* logits);
 

@@ -66,6 +66,11 @@ typedef struct pcbr_options {
     double ml_margin_mm;      /* the learned window: the corridor's bounding box plus this */
     int32_t legalize;         /* after negotiation, reroute what still conflicts with others' copper as obstacles */
     int32_t verbose;
+    int32_t negotiate;        /* 0: waves commit in order with validation (the routes equal one net at a time);
+                                 1: from the second negotiation round on, the nets of a batch (|wave| at once) are
+                                 routed against the board as the batch began and all committed, overlaps left to the
+                                 next round (parallel PathFinder); 2: also in the first round. Legalizing is always
+                                 one net at a time. */
 } pcbr_options;
 void pcbr_default_options(pcbr_options* o);
 

@@ -125,6 +125,7 @@ int64_t pcbgpu_backtrace(const pcbgpu_query* q, const pcbgpu_out* out, int32_t v
 const char* pcbgpu_last_error(void) { return g_err.c_str(); }
 
 int pcbprior_predict(pcbprior*, const float*, int, int, int, float*) { return -1; }
+int pcbprior_predict_valid(pcbprior*, const float*, int, int, int, const int32_t*, float*) { return -1; }
 const char* pcbprior_last_error(void) { return "no network in the oracle build"; }
 
 }  // extern "C"

@@ -83,7 +83,9 @@ __global__ void k_to_bf16(const float* x, uint16_t* out, size_t npix, int cin) {
 // once; a wavefront then holds 4 x 2 accumulator tiles (64 pixels x 32 channels) and per K step reads 4 A and 2 B
 // fragments from LDS for 8 matrix-core products. The results go through LDS so that each lane finishes one pixel:
 // bias, ReLU, the mask (zero where the mask activation is zero: the derivative of the layer before's ReLU), then
-// coalesced BF16 stores of its 32 slots, or FP32 logits of the first cout channels.
+// coalesced BF16 stores of its 32 slots, or FP32 logits of the first cout channels. With `valid` (rows, columns per
+// image) the outputs outside image n's valid extent are 0, so an image padded into a larger batch is computed exactly
+// as alone at its valid size (whose conv reads zeros past its edges).
 constexpr int CR = 4;         // rows per block
 constexpr int XSP = 36;       // LDS pitch of a staged input pixel (BF16): 18 dwords, conflict-free 8-byte reads
 constexpr int WSP = KK + 8;   // LDS pitch of a staged weight row
@@ -91,7 +93,8 @@ constexpr int OSP = C + 1;    // LDS pitch of an output pixel (FP32)
 constexpr int XS_N = (CR + 2) * (PX + 2) * XSP, WS_N = C * WSP;  // BF16 elements
 static_assert(CR * PX * OSP * 2 <= XS_N + WS_N, "the output tile reuses the staging space");
 __global__ __launch_bounds__(256) void k_conv(const uint16_t* x, const uint16_t* wt, const float* bias, uint16_t* ybf,
-                                              float* yf, const uint16_t* mask, int relu, int cout, int H, int W) {
+                                              float* yf, const uint16_t* mask, int relu, int cout, int H, int W,
+                                              const int32_t* valid) {
     __shared__ __align__(16) uint16_t sm[XS_N + WS_N];
     uint16_t* XS = sm;
     uint16_t* WS = sm + XS_N;
@@ -142,10 +145,11 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* x, const uint16_t*
     if (y >= H) return;
     const size_t p = img + size_t(y) * W + x0 + lane;  // W is a multiple of 64: every lane's pixel is inside
     const float* o = O + lane * OSP;
+    const bool inside = !valid || (y < valid[2 * n] && x0 + lane < valid[2 * n + 1]);
     if (yf) {
         for (int co = 0; co < cout; ++co) {
             float v = o[co] + (bias ? bias[co] : 0.f);
-            yf[p * cout + co] = relu ? fmaxf(v, 0.f) : v;
+            yf[p * cout + co] = inside ? (relu ? fmaxf(v, 0.f) : v) : 0.f;
         }
     }
     if (ybf) {
@@ -168,6 +172,7 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* x, const uint16_t*
                         const uint32_t mw = (&mk[q].x)[h];
                         if (((mw >> (16 * k)) & 0x7FFF) == 0) v = 0.f;  // the ReLU was off there
                     }
+                    if (!inside) v = 0.f;
                     pair |= uint32_t(f2bf(v)) << (16 * k);
                 }
                 w[h] = pair;
@@ -375,6 +380,7 @@ struct pcbprior {
     DB<uint16_t> dya, dyb;
     DB<float> logits, tgt, msk, xin, loss;
     DB<unsigned> cnt;
+    DB<int32_t> valid;  // per image (rows, columns) of a padded batch
     int step = 0;
     size_t npix = 0;
     // the resident training set: inputs in BF16 slots, targets and masks, all in HBM
@@ -456,7 +462,7 @@ void collect(pcbprior* p) {  // after a stream sync
 }
 
 // the layers over act[0] (already on the device)
-bool forward_dev(pcbprior* p, int n, int h, int w) {
+bool forward_dev(pcbprior* p, int n, int h, int w, const int32_t* valid = nullptr) {
     const dim3 grid(w / PX, (h + CR - 1) / CR, n);
     const double np = double(n) * h * w;
     for (int L = 0; L < p->depth; ++L) {
@@ -464,17 +470,21 @@ bool forward_dev(pcbprior* p, int n, int h, int w) {
         Timed t(p, 0, 2.0 * np * p->lout[L] * 9 * p->lin[L]);
         hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, p->s, p->act[L].p, p->wt.p + size_t(L) * C * KK,
                            p->bias.p + L * C, last ? nullptr : p->act[L + 1].p, last ? p->logits.p : nullptr,
-                           static_cast<const uint16_t*>(nullptr), last ? 0 : 1, p->cout, h, w);
+                           static_cast<const uint16_t*>(nullptr), last ? 0 : 1, p->cout, h, w, valid);
     }
     HIPOK(hipGetLastError());
     return true;
 }
 
-bool forward(pcbprior* p, const float* x, int n, int h, int w) {
+bool forward(pcbprior* p, const float* x, int n, int h, int w, const int32_t* valid = nullptr) {
     if (!ensure(p, n, h, w)) return false;
     const size_t np = p->npix;
     HIPOK(hipMemcpyAsync(p->xin.p, x, np * p->cin * 4, hipMemcpyHostToDevice, p->s));
     hipLaunchKernelGGL(k_to_bf16, dim3(g1(np * C)), dim3(256), 0, p->s, p->xin.p, p->act[0].p, np, p->cin);
+    if (valid) {
+        if (!p->valid.need(size_t(2) * n) || !h2d(p, p->valid.p, valid, size_t(2) * n * 4)) return false;
+        return forward_dev(p, n, h, w, p->valid.p);
+    }
     return forward_dev(p, n, h, w);
 }
 
@@ -504,7 +514,7 @@ bool backward_dev(pcbprior* p, int n, int h, int w) {
             Timed t(p, 0, fl);
             hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, p->s, dy, p->wf.p + size_t(L) * C * KK,
                                static_cast<const float*>(nullptr), dx, static_cast<float*>(nullptr), p->act[L].p, 0, 0,
-                               h, w);
+                               h, w, static_cast<const int32_t*>(nullptr));
             std::swap(dy, dx);
         }
     }
@@ -671,9 +681,19 @@ int pcbprior_logits(pcbprior* p, const float* x, int n, int h, int w, float* log
 }
 
 int pcbprior_predict(pcbprior* p, const float* x, int n, int h, int w, float* prob) {
+    return pcbprior_predict_valid(p, x, n, h, w, nullptr, prob);
+}
+
+int pcbprior_predict_valid(pcbprior* p, const float* x, int n, int h, int w, const int32_t* hw, float* prob) {
     g_err.clear();
     if (!p || !x || !prob) return -1;
-    if (!forward(p, x, n, h, w)) return -1;
+    if (hw)
+        for (int k = 0; k < n; ++k)
+            if (hw[2 * k] < 1 || hw[2 * k] > h || hw[2 * k + 1] < 1 || hw[2 * k + 1] > w || hw[2 * k + 1] % PX) {
+                g_err = "valid extents: 1 <= rows <= h, columns a multiple of 64 up to w";
+                return -1;
+            }
+    if (!forward(p, x, n, h, w, hw)) return -1;
     const size_t nz = p->npix * p->cout;
     hipLaunchKernelGGL(k_sigmoid, dim3(g1(nz)), dim3(256), 0, p->s, p->logits.p, p->logits.p, nz);
     if (hipMemcpyAsync(prob, p->logits.p, nz * 4, hipMemcpyDeviceToHost, p->s) != hipSuccess ||

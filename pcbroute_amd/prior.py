@@ -30,6 +30,7 @@ def _bind(lib):
     lib.pcbprior_set_params.argtypes = [C.c_void_p, _F]
     for f in (lib.pcbprior_predict, lib.pcbprior_logits):
         f.argtypes = [C.c_void_p, _F, C.c_int, C.c_int, C.c_int, _F]
+    lib.pcbprior_predict_valid.argtypes = [C.c_void_p, _F, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32), _F]
     lib.pcbprior_loss_grad.argtypes = [C.c_void_p, _F, _F, _F, C.c_int, C.c_int, C.c_int, _F, _F]
     lib.pcbprior_train_step.argtypes = [C.c_void_p, _F, _F, _F, C.c_int, C.c_int, C.c_int, C.c_float, _F]
     lib.pcbprior_mfma_check.argtypes = [_F, _F, _F]
@@ -148,6 +149,20 @@ class PriorNet:
         out = np.zeros((n, h, w, self.out_ch), np.float32)
         self._check(self.lib.pcbprior_predict(self.h, _p(x), n, h, w, _p(out)))
         return out
+
+    def predict_many(self, xs):
+        """Images of different sizes (h, w, in_ch), w a multiple of 64, in one padded batch
+        (pcbprior_predict_valid); each result equals predict on the image alone."""
+        H, W = max(x.shape[0] for x in xs), max(x.shape[1] for x in xs)
+        batch = np.zeros((len(xs), H, W, self.in_ch), np.float32)
+        hw = np.zeros(2 * len(xs), np.int32)
+        for k, x in enumerate(xs):
+            batch[k, :x.shape[0], :x.shape[1]] = x
+            hw[2 * k:2 * k + 2] = x.shape[:2]
+        out = np.zeros((len(xs), H, W, self.out_ch), np.float32)
+        self._check(self.lib.pcbprior_predict_valid(self.h, _p(batch), len(xs), H, W,
+                                                    hw.ctypes.data_as(C.POINTER(C.c_int32)), _p(out)))
+        return [out[k, :x.shape[0], :x.shape[1]] for k, x in enumerate(xs)]
 
     def _tm(self, x, target, mask):
         x = self._x(x)

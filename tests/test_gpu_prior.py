@@ -174,3 +174,21 @@ def test_resident_training_matches_host_batches(prior):
         b.train_resident(np.array([5], np.int32))  # out of range
     a.close()
     b.close()
+
+
+def test_padded_batch_equals_each_image_alone(prior):
+    """pcbprior_predict_valid: images of different sizes padded into one batch give exactly the probabilities each
+    gives alone (the valid extent zeroes every layer's activations outside it, as the zero padding of a lone image)."""
+    rng = np.random.default_rng(31)
+    net, _ = _net(prior, rng, 22, 32, 6, 4)
+    xs = [rng.random((h, w, 22)).astype(np.float32) for h, w in ((13, 64), (40, 128), (7, 192), (40, 64))]
+    alone = [net.predict(x[None])[0] for x in xs]
+    many = net.predict_many(xs)
+    for a, b in zip(alone, many):
+        assert a.shape == b.shape and np.array_equal(a, b)
+    lib = prior.library()
+    out = np.zeros(4 * 64 * 4, np.float32)
+    bad = np.array([5, 60], np.int32)  # columns not a multiple of 64
+    assert lib.pcbprior_predict_valid(net.h, prior._p(np.zeros(4 * 64 * 22, np.float32)), 1, 4, 64,
+                                      bad.ctypes.data_as(prior.C.POINTER(prior.C.c_int32)), prior._p(out)) == -1
+    net.close()
