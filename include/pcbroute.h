@@ -105,6 +105,17 @@ void pcbr_set_search_callback(pcbr_router* r, pcbr_search_cb cb, void* user);
 #define PCBR_PIX 4             /* a prior pixel is 4 x 4 grid cells */
 /* attach a network (in_ch = PCBR_FEAT_PER_LAYER * layers + PCBR_FEAT_NET, out_ch = layers); NULL detaches */
 int pcbr_set_prior(pcbr_router* r, pcbprior* net);
+/* The prior file (MLP1; read src/mlprior.cpp:46-74, write src/mlinfer.cpp:1556-1577): "MLP1", f64 pitch, x0, y0
+ * (micrometres), u32 layer count and per layer its name (u32 length, bytes), u32 net count, then per net its name,
+ * i32 i0, j0, u32 w, h and w * h * layers bytes (layer-major, then rows, then columns) over pixels [i0, i0 + w) x
+ * [j0, j0 + h). Here net k is named "N<k>", layer l "L<l>", the board's corner is (0, 0) and a pixel is PCBR_PIX
+ * cells. pcbr_write_prior writes the priors the network gave the last route's first round (nets in name order);
+ * returns the net count or -1. pcbr_read_prior loads a file whose maps then stand in for a network (while none is
+ * attached) in the first round of each route: cells map to the file's pixels and layers match by name as in
+ * src/router.cpp:660-687 (outside a net's map 0, a layer the file lacks no say); nets without a map route plainly.
+ * Returns the number of the board's nets the file has a map for, or -1 (bad magic, pitch, sizes, a short file). */
+int pcbr_write_prior(const pcbr_router* r, const char* path);
+int pcbr_read_prior(pcbr_router* r, const char* path);
 /* Training samples: when enabled, the first route of every net in the first round records its input features
  * (the board as that net's search saw it) over a crop around its pins; after pcbr_route, the net's final route is
  * the target. Crops are ch x cw pixels (cw a multiple of 64), placed around the pins' box and inside the board. */

@@ -66,6 +66,8 @@ def _bind(lib):
     lib.pcbr_net_route.restype = C.c_int64
     lib.pcbr_set_search_callback.argtypes = [C.c_void_p, SEARCH_CB, C.c_void_p]
     lib.pcbr_set_prior.argtypes = [C.c_void_p, C.c_void_p]
+    lib.pcbr_write_prior.argtypes = [C.c_void_p, C.c_char_p]
+    lib.pcbr_read_prior.argtypes = [C.c_void_p, C.c_char_p]
     lib.pcbr_record_samples.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib.pcbr_num_samples.argtypes = [C.c_void_p]
     lib.pcbr_num_samples.restype = C.c_int32
@@ -180,6 +182,21 @@ class Router:
         """Attach a pcbroute_amd.prior.PriorNet (in_ch = 5 x layers + 2, out_ch = layers), or None."""
         self._prior = net
         self.lib.pcbr_set_prior(self.h, net.h if net is not None else None)
+
+    def write_prior(self, path):
+        """The priors of the last route's first round as an MLP1 file; returns the number of nets written."""
+        n = self.lib.pcbr_write_prior(self.h, str(path).encode())
+        if n < 0:
+            raise BackendError(self.lib.pcbr_last_error().decode())
+        return n
+
+    def read_prior(self, path):
+        """Load an MLP1 prior file (used in the first round while no network is attached); returns the number of
+        the board's nets it has a map for."""
+        n = self.lib.pcbr_read_prior(self.h, str(path).encode())
+        if n < 0:
+            raise BackendError(self.lib.pcbr_last_error().decode())
+        return n
 
     def record_samples(self, ch, cw):
         if self.lib.pcbr_record_samples(self.h, ch, cw) != 0:

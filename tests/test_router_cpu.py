@@ -122,3 +122,86 @@ def test_training_samples(lib):
         mst = x[k, :, :, R.FEAT_PER_LAYER * nl]
         assert mst[own_pads > 0].max() > 0.8                      # the spanning tree passes through the pins
         assert t[k].sum() > 0 and (t[k].max(axis=-1) * mst).sum() > 0  # the route lies along it
+
+
+def _mlp1(path, pitch, x0, y0, layers, maps):
+    """An MLP1 file written from its published layout (src/mlprior.cpp:46): magic, f64 pitch, x0, y0, u32 layer
+    count and names, u32 net count, per net name, i32 i0, j0, u32 w, h, u8 maps layer-major."""
+    import struct
+    with open(path, "wb") as f:
+        f.write(b"MLP1" + struct.pack("<3d", pitch, x0, y0) + struct.pack("<I", len(layers)))
+        for name in layers:
+            f.write(struct.pack("<I", len(name)) + name.encode())
+        f.write(struct.pack("<I", len(maps)))
+        for name, (i0, j0, data) in sorted(maps.items()):
+            nl, h, w = data.shape
+            f.write(struct.pack("<I", len(name)) + name.encode() + struct.pack("<iiII", i0, j0, w, h))
+            f.write(np.ascontiguousarray(data, np.uint8).tobytes())
+
+
+@pytest.mark.parametrize("pitch_cells", [4, 1.5])
+def test_prior_file_of_full_maps_routes_as_plain(lib, tmp_path, pitch_cells):
+    """Maps of 255 over the whole board: no bias (mlbias[255] = 0) and a corridor covering every window, so the
+    learned searches are the plain ones, at the router's own pixel pitch and at a foreign one."""
+    b = R.Board(R.spec("tiny", seed=2), lib_path=lib)
+    i = b.info
+    cell_um = R.spec("tiny", seed=2).cell_um
+    pitch = pitch_cells * cell_um
+    w, h = int(np.ceil(i.nx / pitch_cells)) + 1, int(np.ceil(i.ny / pitch_cells)) + 1
+    full = np.full((i.nl, h, w), 255, np.uint8)
+    path = str(tmp_path / "full.mlp")
+    _mlp1(path, pitch, 0.0, 0.0, [f"L{l}" for l in range(i.nl)], {f"N{n}": (0, 0, full) for n in range(i.n_nets)})
+    plain = R.Router(b, R.default_options(R.library(lib)), lib_path=lib).route()
+    r = R.Router(b, R.default_options(R.library(lib)), lib_path=lib)
+    assert r.read_prior(path) == i.n_nets
+    res = r.route()
+    assert res["route_hash"] == plain["route_hash"] and res["violations"] == 0
+
+
+def test_prior_file_layers_match_by_name(lib, tmp_path):
+    """A file whose layer names match none of the board's has no say on any layer: plain routes; nets without a map
+    route plainly too."""
+    b = R.Board(R.spec("tiny", seed=2), lib_path=lib)
+    i = b.info
+    zero = np.zeros((1, 200, 200), np.uint8)  # would forbid everything, were its layer the board's
+    path = str(tmp_path / "other.mlp")
+    _mlp1(path, 200.0, 0.0, 0.0, ["F.Cu"], {"N0": (0, 0, zero), "N999": (0, 0, zero)})
+    plain = R.Router(b, R.default_options(R.library(lib)), lib_path=lib).route()
+    r = R.Router(b, R.default_options(R.library(lib)), lib_path=lib)
+    assert r.read_prior(path) == 1  # N0 only: N999 is not a net of this board
+    assert r.route()["route_hash"] == plain["route_hash"]
+
+
+def test_prior_file_errors(lib, tmp_path):
+    b = R.Board(R.spec("tiny", seed=2), lib_path=lib)
+    r = R.Router(b, R.default_options(R.library(lib)), lib_path=lib)
+    good = str(tmp_path / "good.mlp")
+    _mlp1(good, 200.0, 0.0, 0.0, ["L0", "L1"], {"N1": (3, 4, np.ones((2, 5, 6), np.uint8))})
+    assert r.read_prior(good) == 1
+    raw = open(good, "rb").read()
+    cases = {"magic": b"MLP2" + raw[4:], "short": raw[:-1], "pitch": raw[:4] + b"\0" * 8 + raw[12:],
+             "nothing": b""}
+    for name, data in cases.items():
+        p = str(tmp_path / f"{name}.mlp")
+        open(p, "wb").write(data)
+        with pytest.raises(R.BackendError):
+            r.read_prior(p)
+    with pytest.raises(R.BackendError):
+        r.read_prior(str(tmp_path / "missing.mlp"))
+    with pytest.raises(R.BackendError):
+        r.write_prior(str(tmp_path / "no" / "dir.mlp"))
+
+
+def test_prior_file_maps_take_effect(lib, tmp_path):
+    """Maps of 0 forbid every cell but the targets: the learned searches find nothing and fall back to the plain
+    search (src/router.cpp:1804), so the file was applied; the board still ends clean."""
+    b = R.Board(R.spec("tiny", seed=2), lib_path=lib)
+    i = b.info
+    zero = np.zeros((i.nl, i.ny // 4 + 1, i.nx // 4 + 1), np.uint8)
+    path = str(tmp_path / "zero.mlp")
+    _mlp1(path, 4 * R.spec("tiny", seed=2).cell_um, 0.0, 0.0, [f"L{l}" for l in range(i.nl)],
+          {f"N{n}": (0, 0, zero) for n in range(i.n_nets)})
+    r = R.Router(b, R.default_options(R.library(lib)), lib_path=lib)
+    r.read_prior(path)
+    res = r.route()
+    assert res["search_fallbacks"] > 0 and res["violations"] == 0

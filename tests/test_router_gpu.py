@@ -99,3 +99,21 @@ def test_learned_prior_gains_over_plain():
         tot["plain"] += plain["opens"]
         tot["learned"] += learned["opens"]
     assert tot["learned"] < tot["plain"]
+
+
+def test_prior_file_round_trip(tmp_path):
+    """The network's priors written as an MLP1 file and read back by another router (no network attached) route the
+    board exactly as the network did: the file carries every map byte and the pixel geometry."""
+    from pcbroute_amd import learn
+    net = learn.load_net()
+    b = R.Board(R.spec("c4", seed=5002))
+    a = R.Router(b)
+    a.set_prior(net)
+    first = a.route()
+    path = str(tmp_path / "c4.mlp")
+    assert a.write_prior(path) == b.info.n_nets
+    r = R.Router(b)
+    assert r.read_prior(path) == b.info.n_nets
+    again = r.route()
+    _say("prior file", first["route_hash"], again["route_hash"], first["opens"], again["opens"])
+    assert again["route_hash"] == first["route_hash"] and again["violations"] == 0
