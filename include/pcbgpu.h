@@ -196,6 +196,9 @@ typedef struct pcbgpu_stats {
     uint64_t rect_state_passes;  /* sum over rectangles of states x passes */
     double kernel_ms;            /* summed duration of the sweep-pass launches (events) */
     double solve_ms;             /* summed wall time of the rectangle solves */
+    uint64_t state_sweeps;       /* states times the local sweeps run over them: the state-sweep of
+                                    src/gpu.cu:493-524 (a local sweep carries every state of a tile through the eight
+                                    directional relaxations and the vias), the algorithmic unit of 12 bytes */
 } pcbgpu_stats;
 void pcbgpu_stats_enable(int on);
 void pcbgpu_stats_read(pcbgpu_stats* s, int reset);

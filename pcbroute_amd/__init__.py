@@ -32,7 +32,7 @@ class Tune(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("solves", C.c_uint64), ("launches", C.c_uint64), ("passes", C.c_uint64), ("tiles", C.c_uint64),
                 ("states", C.c_uint64), ("rect_state_passes", C.c_uint64), ("kernel_ms", C.c_double),
-                ("solve_ms", C.c_double)]
+                ("solve_ms", C.c_double), ("state_sweeps", C.c_uint64)]
 
 
 class BackendError(RuntimeError):

@@ -302,15 +302,18 @@ __global__ __launch_bounds__(256) void k_count(const float* m, size_t npix, unsi
 // a batch from the resident training set: sample idx[b] into slot b (inputs in BF16 slots, targets, mask)
 __global__ void k_gather(const uint16_t* dsx, const float* dst, const float* dsm, const int32_t* idx, size_t spix,
                          int cout, int batch, uint16_t* x, float* t, float* m) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;  // over batch * spix * C
-    if (i >= size_t(batch) * spix * C) return;
-    const size_t b = i / (spix * C), r = i % (spix * C), src = size_t(idx[b]) * spix * C + r;
-    x[i] = dsx[src];
-    if (r % C == 0) {
-        const size_t px = r / C, o = size_t(idx[b]) * spix + px, dpx = b * spix + px;
-        m[dpx] = dsm[o];
-        for (int c = 0; c < cout; ++c) t[dpx * cout + c] = dst[o * cout + c];
-    }
+    // blockIdx.y: the batch slot; each sample is one contiguous run per array, copied in 16-byte vectors (a pixel's
+    // C BF16 slots are 64 bytes) and grid-stride loops
+    const int b = blockIdx.y;
+    const size_t s = size_t(idx[b]);
+    const size_t i0 = size_t(blockIdx.x) * blockDim.x + threadIdx.x, step = size_t(gridDim.x) * blockDim.x;
+    const uint4* xs = reinterpret_cast<const uint4*>(dsx + s * spix * C);
+    uint4* xd = reinterpret_cast<uint4*>(x + size_t(b) * spix * C);
+    for (size_t i = i0; i < spix * C / 8; i += step) xd[i] = xs[i];
+    const float* ts = dst + s * spix * cout;
+    float* td = t + size_t(b) * spix * cout;
+    for (size_t i = i0; i < spix * cout; i += step) td[i] = ts[i];
+    for (size_t i = i0; i < spix; i += step) m[size_t(b) * spix + i] = dsm[s * spix + i];
 }
 
 __global__ void k_sigmoid(const float* z, float* p, size_t n) {
@@ -557,7 +560,8 @@ bool gather(pcbprior* p, const int32_t* idx, int batch) {
     if (!ensure(p, batch, p->ds_h, p->ds_w) || !p->didx.need(size_t(batch)) || !h2d(p, p->didx.p, idx, size_t(batch) * 4))
         return false;
     const size_t spix = size_t(p->ds_h) * p->ds_w;
-    hipLaunchKernelGGL(k_gather, dim3(g1(size_t(batch) * spix * C)), dim3(256), 0, p->s, p->dsx.p, p->dst.p, p->dsm.p,
+    const unsigned gx = unsigned(std::min<size_t>(512, (spix * C / 8 + 255) / 256));
+    hipLaunchKernelGGL(k_gather, dim3(gx, unsigned(batch)), dim3(256), 0, p->s, p->dsx.p, p->dst.p, p->dsm.p,
                        p->didx.p, spix, p->cout, batch, p->act[0].p, p->tgt.p, p->msk.p);
     HIPOK(hipGetLastError());
     return true;
