@@ -82,11 +82,11 @@ def corridor_stats(net, data, thr=0.05):
     return recall, kept
 
 
-CONFIGS = {  # name: (wave, group_mm, prior)
+CONFIGS = {  # name: (wave, group_mm, prior); every configuration's routes equal routing its order one net at a time
     "plain": (0, 1.0, False),
-    "batched": (16, 1.0, False),
     "learned": (0, 1.0, True),
-    "batched_learned": (16, 1.0, True),
+    "batched_learned": (16, 1.0, True),    # speculative pipeline, commits validated in order
+    "grouped_learned": (-64, 1.0, True),   # box-disjoint groups routed together, commits validated in order
 }
 FIELDS = ("opens", "violations", "route_s", "search_s", "prior_s", "searches", "rounds", "revalidated", "iterations",
           "search_fallbacks", "length_cells", "vias")
