@@ -182,17 +182,43 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* x, const uint16_t*
     }
 }
 
-// dW[co][k] += sum over the block's pixels of dy[p][co] * x_im2col[p][k]; db[co] += sum of dy[p][co]. A block takes
-// 64 columns of WG_ROWS rows, so that the FP32 atomics into dW are few next to the products. Per row it stages the
-// three input rows it needs (66 pixels with the halo) and the dy row in LDS; wavefront w then takes the k tiles w,
-// w + 4, ... against both co tiles. The reduction runs over pixels, so a lane's 4 operand values are 4 pixels of one
-// channel: LDS reads of one BF16 each.
-constexpr int WG_ROWS = 8;
-constexpr int XP = C + 2;  // LDS pitch of a pixel (BF16), padded against bank conflicts
+// dW[co][k] += sum over the block's pixels of dy[p][co] * x_im2col[p][k]; db[co] += sum of dy[p][co]. The reduction
+// runs over pixels, so a lane's 4 operand values are 4 consecutive pixels of one channel: the block stages its output
+// gradient and its input rows transposed in LDS (a channel's pixels contiguous), and a fragment is one aligned 8-byte
+// read (two, and a byte shift, where the 3x3 tap's column offset misaligns it). A block takes 64 columns of WG_ROWS
+// rows in chunks of WR rows (WR + 2 input rows staged per chunk); wavefront w accumulates the k tiles w, w + 4, ...
+// against both co tiles in registers and adds them into dW with FP32 atomics once.
+constexpr int WG_ROWS = 16, WR = 4;
+constexpr int DTP = WR * PX + 4;         // LDS pitch of a channel of the staged output gradient (BF16)
+constexpr int XTW = PX + 8;              // a staged input row: columns xb - 4 .. xb + 67, column xb + c at c + 4
+constexpr int XTP = (WR + 2) * XTW + 4;  // LDS pitch of a channel of the staged input
+__device__ inline v4s ld_px4(const uint16_t* row, int pos, int sh) {  // row[pos .. pos + 3]; sh = pos & 3, uniform
+    const uint2 lo = *reinterpret_cast<const uint2*>(row + (pos & ~3));
+    uint32_t r0 = lo.x, r1 = lo.y;
+    if (sh) {
+        const uint2 hi = *reinterpret_cast<const uint2*>(row + (pos & ~3) + 4);
+        if (sh == 1) { r0 = __builtin_amdgcn_alignbyte(lo.y, lo.x, 2); r1 = __builtin_amdgcn_alignbyte(hi.x, lo.y, 2); }
+        else if (sh == 2) { r0 = lo.y; r1 = hi.x; }
+        else { r0 = __builtin_amdgcn_alignbyte(hi.x, lo.y, 2); r1 = __builtin_amdgcn_alignbyte(hi.y, hi.x, 2); }
+    }
+    v4s v;
+    v[0] = short(r0 & 0xFFFF); v[1] = short(r0 >> 16); v[2] = short(r1 & 0xFFFF); v[3] = short(r1 >> 16);
+    return v;
+}
+// 4 pixels x 8 channels (four 16-byte loads, pixel-major) into 8 runs of 4 pixels of one channel (8-byte stores)
+__device__ inline void put4(uint16_t* dst, int pitch, const uint4 (&v)[4]) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint32_t e[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) e[j] = ((&v[j].x)[k >> 1] >> (16 * (k & 1))) & 0xFFFF;
+        *reinterpret_cast<uint2*>(dst + k * pitch) = make_uint2(e[0] | (e[1] << 16), e[2] | (e[3] << 16));
+    }
+}
 __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* x, const uint16_t* dy, float* dw, float* db, int H, int W) {
-    __shared__ uint16_t xs[3][PX + 2][XP];
-    __shared__ uint16_t ds[PX][XP];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ __align__(16) uint16_t DT[C * DTP];
+    __shared__ __align__(16) uint16_t XT[C * XTP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int n = blockIdx.z, xb = blockIdx.x * PX;
     const int pr = lane & 15, kq = lane >> 4;
     const size_t img = size_t(n) * H * W;
@@ -203,48 +229,56 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* x, const uint16_t
 #pragma unroll
         for (int u = 0; u < 5; ++u) acc[t][u] = v4f{0, 0, 0, 0};
     float bsum = 0.f;
-    const int y0 = blockIdx.y * WG_ROWS, y1 = min(H, y0 + WG_ROWS);
-    for (int y = y0; y < y1; ++y) {
-        // stage: 4 channels (8 bytes) per thread and step
-        for (int e = threadIdx.x; e < 3 * (PX + 2) * (C / 4); e += 256) {
-            const int r = e / ((PX + 2) * (C / 4)), rem = e % ((PX + 2) * (C / 4)), q = rem / (C / 4), c4 = rem % (C / 4);
-            const int py = y + r - 1, px = xb + q - 1;
-            uint2 v = make_uint2(0, 0);
-            if (py >= 0 && py < H && px >= 0 && px < W) v = *reinterpret_cast<const uint2*>(x + (img + size_t(py) * W + px) * C + c4 * 4);
-            uint16_t* d = &xs[r][q][c4 * 4];
-            d[0] = uint16_t(v.x & 0xFFFF); d[1] = uint16_t(v.x >> 16); d[2] = uint16_t(v.y & 0xFFFF); d[3] = uint16_t(v.y >> 16);
+    const int y1 = min(H, (blockIdx.y + 1) * WG_ROWS);
+    for (int c0 = blockIdx.y * WG_ROWS; c0 < y1; c0 += WR) {
+        const int rows = min(WR, y1 - c0);
+        for (int e = tid; e < WR * (PX / 4) * (C / 8); e += 256) {  // output gradient rows c0 .. c0 + WR - 1
+            const int r = e / ((PX / 4) * (C / 8)), rem = e % ((PX / 4) * (C / 8)), g = rem / (C / 8), c8 = rem % (C / 8);
+            uint4 v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                v[j] = r < rows ? *reinterpret_cast<const uint4*>(dy + (img + size_t(c0 + r) * W + xb + 4 * g + j) * C + c8 * 8)
+                                : make_uint4(0, 0, 0, 0);
+            put4(DT + (c8 * 8) * DTP + r * PX + 4 * g, DTP, v);
         }
-        for (int e = threadIdx.x; e < PX * (C / 4); e += 256) {
-            const int q = e / (C / 4), c4 = e % (C / 4), px = xb + q;
-            uint2 v = make_uint2(0, 0);
-            if (px < W) v = *reinterpret_cast<const uint2*>(dy + (img + size_t(y) * W + px) * C + c4 * 4);
-            uint16_t* d = &ds[q][c4 * 4];
-            d[0] = uint16_t(v.x & 0xFFFF); d[1] = uint16_t(v.x >> 16); d[2] = uint16_t(v.y & 0xFFFF); d[3] = uint16_t(v.y >> 16);
-        }
-        __syncthreads();
-        if (threadIdx.x < C)
-            for (int q = 0; q < PX; ++q) bsum += bf2f(ds[q][threadIdx.x]);
-        for (int ps = 0; ps < PX / 16; ++ps) {
-            // A = dy^T (row co, column pixel), B = x_im2col (row pixel, column k)
-            v4s a0, a1;
+        for (int e = tid; e < (WR + 2) * (XTW / 4) * (C / 8); e += 256) {  // input rows c0 - 1 .. c0 + WR
+            const int r = e / ((XTW / 4) * (C / 8)), rem = e % ((XTW / 4) * (C / 8)), g = rem / (C / 8), c8 = rem % (C / 8);
+            const int py = c0 - 1 + r;
+            uint4 v[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                a0[j] = short(ds[ps * 16 + kq * 4 + j][pr]);
-                a1[j] = short(ds[ps * 16 + kq * 4 + j][16 + pr]);
+                const int px = xb - 4 + 4 * g + j;
+                v[j] = (py >= 0 && py < H && px >= 0 && px < W)
+                           ? *reinterpret_cast<const uint4*>(x + (img + size_t(py) * W + px) * C + c8 * 8)
+                           : make_uint4(0, 0, 0, 0);
             }
+            put4(XT + (c8 * 8) * XTP + r * XTW + 4 * g, XTP, v);
+        }
+        __syncthreads();
+        if (tid < C)
+            for (int q = 0; q < rows * PX; q += 4) {
+                const v4s d4 = ld_px4(DT + tid * DTP, q, 0);
 #pragma unroll
-            for (int u = 0; u < 5; ++u) {
-                const int kt = wave + 4 * u;
-                if (kt < KT) {
-                    const int kc = kt * 16 + pr, d = kc / C, ci = kc % C, r = d / 3, dxx = d % 3;  // halo: q = px + dxx
-                    v4s b;
+                for (int j = 0; j < 4; ++j) bsum += bf2f(uint16_t(d4[j]));
+            }
+        for (int r = 0; r < rows; ++r)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) b[j] = short(xs[r][ps * 16 + kq * 4 + j + dxx][ci]);
-                    acc[0][u] = mfma16(a0, b, acc[0][u]);
-                    acc[1][u] = mfma16(a1, b, acc[1][u]);
+            for (int ps = 0; ps < PX / 16; ++ps) {
+                // A = dy^T (row co, column pixel), B = x_im2col (row pixel, column k)
+                const int p0 = ps * 16 + kq * 4;
+                const v4s a0 = ld_px4(DT + pr * DTP + r * PX, p0, 0), a1 = ld_px4(DT + (16 + pr) * DTP + r * PX, p0, 0);
+#pragma unroll
+                for (int u = 0; u < 5; ++u) {
+                    const int kt = wave + 4 * u;
+                    if (kt < KT) {
+                        // k = kt * 16 + pr: tap d = k / C (the same for the whole k tile), channel ci = k % C
+                        const int d = kt >> 1, ci = (kt & 1) * 16 + pr, dyy = d / 3, dxx = d % 3;
+                        const v4s bb = ld_px4(XT + ci * XTP + (r + dyy) * XTW, p0 + dxx + 3, (dxx + 3) & 3);
+                        acc[0][u] = mfma16(a0, bb, acc[0][u]);
+                        acc[1][u] = mfma16(a1, bb, acc[1][u]);
+                    }
                 }
             }
-        }
         __syncthreads();
     }
 #pragma unroll
@@ -256,7 +290,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* x, const uint16_t
 #pragma unroll
                 for (int j = 0; j < 4; ++j) atomicAdd(&dw[size_t(16 * t + kq * 4 + j) * KK + kt * 16 + pr], acc[t][u][j]);
         }
-    if (threadIdx.x < C) atomicAdd(&db[threadIdx.x], bsum);
+    if (tid < C) atomicAdd(&db[tid], bsum);
 }
 
 // binary cross-entropy of the logits over the masked pixels; dz = (sigmoid(z) - t) m / count into the BF16 slots
