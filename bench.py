@@ -161,6 +161,69 @@ def round_workload(b, a):
 PRIOR_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix cores
 
 
+def _ref_eval(ref, dsn, ses):
+    import re
+    import subprocess
+    out = subprocess.run([ref, "-de", dsn, "--eval", ses], capture_output=True, text=True, timeout=600)
+    m = re.search(r"eval: (\d+) nets, (\d+) connections, (\d+) open, (\d+) violations, ([\d.]+) mm of track, (\d+) vias",
+                  out.stdout)
+    if not m:
+        return None
+    k = ("nets", "connections", "open", "violations", "track_mm", "vias")
+    return {n: (float(v) if n == "track_mm" else int(v)) for n, v in zip(k, m.groups())}
+
+
+def c1_workload(b, a):
+    """C1: the seeded 40x30 mm 2-layer board (50 nets) routed by the router on the GPU, one net at a time with full
+    negotiation; the routes written as a session and judged by the reference's own evaluator; the reference's CPU
+    router (oracle/_ref/pcbroute, built from its sources) routing the same board, written as DSN, timed beside it."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    from pcbroute_amd import router as R
+    board = R.Board(R.spec("c1", seed=a.board_seed))
+    times, res, r = [], None, None
+    for k in range(a.warmup + a.steps):
+        if r is not None:
+            r.close()
+        r = R.Router(board)
+        t0 = time.perf_counter()
+        res = r.route()
+        if k >= a.warmup:
+            times.append(time.perf_counter() - t0)
+        log(f"c1: route {k + 1}/{a.warmup + a.steps} done")
+    ref = os.path.join(ROOT, "oracle", "_ref", "pcbroute")
+    line = {
+        "metric": "C1 board route time", "value": round(float(np.median(times)), 4), "unit": "s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(float(np.median(times)) * 1e3, 1),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "C1: seeded 40x30 mm 2-layer board, 6 SOIC-16 + 20 0805, 50 nets of 2-6 pins, "
+                               "PathFinder, one net at a time", "board_seed": a.board_seed},
+        "result": {k: res[k] for k in ("nets", "connections", "opens", "violations", "iterations", "searches",
+                                       "length_cells", "vias", "route_hash")},
+    }
+    with tempfile.TemporaryDirectory() as d:
+        dsn, ses = os.path.join(d, "c1.dsn"), os.path.join(d, "c1.ses")
+        board.write_dsn(dsn)
+        r.write_ses(ses)
+        if os.path.exists(ref):
+            line["reference_eval_of_our_session"] = _ref_eval(ref, dsn, ses)
+            if not a.no_cpu:
+                threads = min(16, os.cpu_count() or 1)
+                out_ses = os.path.join(d, "ref.ses")
+                t0 = time.perf_counter()
+                p = subprocess.run([ref, "-de", dsn, "-do", out_ses, "-mt", str(threads)], capture_output=True,
+                                   text=True, timeout=1200)
+                t_ref = time.perf_counter() - t0
+                q = _ref_eval(ref, dsn, out_ses) if p.returncode == 0 else None
+                line["cpu_baseline"] = {"value": round(t_ref, 3), "unit": "s", "cores": threads, "kind": "reference",
+                                        "sample": "the reference's CPU router (built from its sources) on the whole "
+                                                  "board written as DSN, wall time of the process",
+                                        "its_routes": q}
+    r.close()
+    print(json.dumps(line), flush=True)
+
+
 def prior_workload(b, a):
     """The learned prior's training and inference on full 4-layer 60x50 mm boards: the 4x-downsampled grid is
     300 x 250 pixels, padded to 320 x 256; 22 input channels, 6 conv3x3 layers of 32 channels, 4 outputs. The
@@ -252,10 +315,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=("both", "single", "round"), default="both")
-    ap.add_argument("--workload", choices=("kernel", "round", "prior"), default="kernel",
+    ap.add_argument("--workload", choices=("kernel", "round", "prior", "c1"), default="kernel",
                     help="kernel: the 64 whole-grid searches (the metric); round: 256 net searches of a speculative "
                          "round on a 2400x1800x6 board with two net classes, as one round and one at a time; "
-                         "prior: the corridor CNN's training and inference on full 4-layer boards")
+                         "prior: the corridor CNN's training and inference on full 4-layer boards; c1: the router on "
+                         "the C1 board, its session judged by the reference's evaluator, the reference's CPU router "
+                         "timed beside it")
     ap.add_argument("--tune", choices=("auto", "off"), default=None,
                     help="auto: the autotuner picks tile rows, local sweeps and delta on a sample drawn with another "
                          "seed (the default for --workload round); off: library defaults or the explicit flags (the "
@@ -293,6 +358,8 @@ def main():
         return round_workload(b, a)
     if a.workload == "prior":
         return prior_workload(b, a)
+    if a.workload == "c1":
+        return c1_workload(b, a)
     log("building the board")
     board = synth.kernel_board(a.board_seed)
     probs = synth.pair_problems(board, a.pairs_seed, a.pairs)

@@ -37,6 +37,15 @@ typedef struct pcbr_board_spec {
 typedef struct pcbr_board pcbr_board;
 pcbr_board* pcbr_board_generate(const pcbr_board_spec* spec);
 void pcbr_board_free(pcbr_board* b);
+/* A board from a Specctra DSN file (src/dsn.cpp:75-384, read_dsn) on a grid of cell_um cells: the signal layers, the
+ * boundary (its bounding box is the grid; a polygon outline blocks what lies outside it), the default rule and the
+ * classes' width, clearance and via, every part's pins placed as src/dsn.cpp:239-252 places them, each pad as the
+ * bounding rectangle of its padstack's shapes, the structure's keepouts as rectangles, the copper-to-edge clearance of
+ * (pcbroute_rules (edge_clearance UM)). Planes and pre-routed wiring are not read. NULL on error (pcbr_last_error). */
+pcbr_board* pcbr_board_read_dsn(const char* path, double cell_um);
+/* A board (generated or read) as a DSN file any DSN router reads: every pad a one-pin part on a rectangle padstack,
+ * the classes with their rules and vias. 0 or -1. */
+int pcbr_board_write_dsn(const pcbr_board* b, const char* path);
 
 typedef struct pcbr_board_info {
     int32_t nx, ny, nl, n_pads, n_nets, n_pins, n_connections, n_parts;
@@ -115,6 +124,10 @@ int pcbr_set_prior(pcbr_router* r, pcbprior* net);
  * src/router.cpp:660-687 (outside a net's map 0, a layer the file lacks no say); nets without a map route plainly.
  * Returns the number of the board's nets the file has a map for, or -1 (bad magic, pitch, sizes, a short file). */
 int pcbr_write_prior(const pcbr_router* r, const char* path);
+/* The routes as a Specctra session file (write_ses, src/router.cpp:2938-2963): per net its wires from cell centre to
+ * cell centre at its class's width, collinear steps merged, and a via of its class at every layer change; the
+ * board's names and coordinates (a generated board: nets N<k>, layers L<l>, vias V<class>). 0 or -1. */
+int pcbr_write_ses(const pcbr_router* r, const char* path);
 int pcbr_read_prior(pcbr_router* r, const char* path);
 /* Training samples: when enabled, the first route of every net in the first round records its input features
  * (the board as that net's search saw it) over a crop around its pins; after pcbr_route, the net's final route is

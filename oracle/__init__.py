@@ -25,7 +25,8 @@ def build_route_oracle(verbose=False):
     obj = os.path.join(HERE, "pcb_oracle_route.o")
     cmds = [["gcc", "-O2", "-std=c11", "-fPIC", "-c", "-o", obj, SRC],
             ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", ROUTE_LIB + ".tmp",
-             os.path.join(root, "pcbroute_amd", "csrc", "router.cpp"), os.path.join(HERE, "route_on_oracle.cpp"), obj,
+             os.path.join(root, "pcbroute_amd", "csrc", "router.cpp"), os.path.join(root, "pcbroute_amd", "csrc", "dsn.cpp"),
+             os.path.join(HERE, "route_on_oracle.cpp"), obj,
              "-lpthread"]]
     for cmd in cmds:
         if verbose:

@@ -12,7 +12,7 @@ ARCH = os.environ.get("PCBGPU_ARCH", "gfx950")
 # packed-fp32 instructions are switched off for every HIP translation unit
 HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-fPIC", "-Wall"]
-SOURCES = ["pcbgpu.hip", "prior_cnn.hip", "backtrace.cpp", "gpu_shim.cpp", "router.cpp"]
+SOURCES = ["pcbgpu.hip", "prior_cnn.hip", "backtrace.cpp", "gpu_shim.cpp", "router.cpp", "dsn.cpp"]
 CHECK = os.path.join(HERE, "shim_check")  # a C++ program that uses the gpu.h shim (built beside the library)
 
 

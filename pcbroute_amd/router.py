@@ -54,6 +54,10 @@ def _bind(lib):
     lib.pcbr_board_generate.argtypes = [C.POINTER(BoardSpec)]
     lib.pcbr_board_generate.restype = C.c_void_p
     lib.pcbr_board_free.argtypes = [C.c_void_p]
+    lib.pcbr_board_read_dsn.argtypes = [C.c_char_p, C.c_double]
+    lib.pcbr_board_read_dsn.restype = C.c_void_p
+    lib.pcbr_board_write_dsn.argtypes = [C.c_void_p, C.c_char_p]
+    lib.pcbr_write_ses.argtypes = [C.c_void_p, C.c_char_p]
     lib.pcbr_board_get_info.argtypes = [C.c_void_p, C.POINTER(BoardInfo)]
     lib.pcbr_board_pads.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_size_t]
     lib.pcbr_default_options.argtypes = [C.POINTER(Options)]
@@ -124,14 +128,26 @@ def default_options(lib=None, **kw):
 
 
 class Board:
-    def __init__(self, spec_, lib_path=LIB_PATH):
+    def __init__(self, spec_, lib_path=LIB_PATH, dsn=None, cell_um=50.0):
+        """A seeded synthetic board of spec_, or (spec_ None) the board of the DSN file dsn on cell_um cells."""
         self.lib = library(lib_path)
         self.spec = spec_
-        self.h = self.lib.pcbr_board_generate(C.byref(spec_))
+        if spec_ is None:
+            self.h = self.lib.pcbr_board_read_dsn(str(dsn).encode(), float(cell_um))
+        else:
+            self.h = self.lib.pcbr_board_generate(C.byref(spec_))
         if not self.h:
             raise BackendError(self.lib.pcbr_last_error().decode())
         self.info = BoardInfo()
         self.lib.pcbr_board_get_info(self.h, C.byref(self.info))
+
+    @classmethod
+    def from_dsn(cls, path, cell_um=50.0, lib_path=LIB_PATH):
+        return cls(None, lib_path=lib_path, dsn=path, cell_um=cell_um)
+
+    def write_dsn(self, path):
+        if self.lib.pcbr_board_write_dsn(self.h, str(path).encode()) != 0:
+            raise BackendError(self.lib.pcbr_last_error().decode())
 
     def pads(self):
         n = self.lib.pcbr_board_pads(self.h, None, 0)
@@ -182,6 +198,11 @@ class Router:
         """Attach a pcbroute_amd.prior.PriorNet (in_ch = 5 x layers + 2, out_ch = layers), or None."""
         self._prior = net
         self.lib.pcbr_set_prior(self.h, net.h if net is not None else None)
+
+    def write_ses(self, path):
+        """The routes as a Specctra session file (after route)."""
+        if self.lib.pcbr_write_ses(self.h, str(path).encode()) != 0:
+            raise BackendError(self.lib.pcbr_last_error().decode())
 
     def write_prior(self, path):
         """The priors of the last route's first round as an MLP1 file; returns the number of nets written."""

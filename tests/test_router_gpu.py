@@ -117,3 +117,26 @@ def test_prior_file_round_trip(tmp_path):
     again = r.route()
     _say("prior file", first["route_hash"], again["route_hash"], first["opens"], again["opens"])
     assert again["route_hash"] == first["route_hash"] and again["violations"] == 0
+
+
+def test_c1_session_clean_by_the_reference(tmp_path):
+    """The C1 board routed on the GPU, its routes written as a session and judged by the reference's own evaluator
+    (oracle/_ref/pcbroute, built from the reference's sources where they are present): every connection made, no
+    violation, the connection count both sides agree on."""
+    import re
+    import subprocess
+    ref = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "pcbroute")
+    if not os.path.exists(ref):
+        pytest.skip("the reference's evaluator is not built")
+    b = R.Board(R.spec("c1", seed=1))
+    d, s = str(tmp_path / "c1.dsn"), str(tmp_path / "c1.ses")
+    b.write_dsn(d)
+    r = R.Router(b)
+    res = r.route()
+    r.write_ses(s)
+    out = subprocess.run([ref, "-de", d, "--eval", s], capture_output=True, text=True, timeout=300)
+    m = re.search(r"eval: (\d+) nets, (\d+) connections, (\d+) open, (\d+) violations", out.stdout)
+    assert m, out.stdout + out.stderr
+    nets, conns, opens, viol = (int(x) for x in m.groups())
+    _say("c1 by the reference:", out.stdout.strip())
+    assert conns == b.info.n_connections and opens == res["opens"] == 0 and viol == 0
