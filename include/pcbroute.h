@@ -40,8 +40,9 @@ void pcbr_board_free(pcbr_board* b);
 /* A board from a Specctra DSN file (src/dsn.cpp:75-384, read_dsn) on a grid of cell_um cells: the signal layers, the
  * boundary (its bounding box is the grid; a polygon outline blocks what lies outside it), the default rule and the
  * classes' width, clearance and via, every part's pins placed as src/dsn.cpp:239-252 places them, each pad as the
- * bounding rectangle of its padstack's shapes, the structure's keepouts as rectangles, the copper-to-edge clearance of
- * (pcbroute_rules (edge_clearance UM)). Planes and pre-routed wiring are not read. NULL on error (pcbr_last_error). */
+ * bounding rectangle of its padstack's shapes, the keepouts of the structure and of the parts' images (placed with
+ * their part) as rectangles, the copper-to-edge clearance of (pcbroute_rules (edge_clearance UM)), else 500 um.
+ * Planes and pre-routed wiring are not read. NULL on error (pcbr_last_error). */
 pcbr_board* pcbr_board_read_dsn(const char* path, double cell_um);
 /* A board (generated or read) as a DSN file any DSN router reads: every pad a one-pin part on a rectangle padstack,
  * the classes with their rules and vias. 0 or -1. */

@@ -196,3 +196,51 @@ def test_dsn_errors(lib, tmp_path):
         R.Board.from_dsn(str(tmp_path / "missing.dsn"), lib_path=lib)
     with pytest.raises(R.BackendError):
         R.Board.from_dsn(_write(tmp_path, "gate.dsn", GATE), cell_um=0, lib_path=lib)
+
+
+# the board of the reference's tests/test_image_keepouts.py: two through-hole pins and a part whose image is a hole
+# (a keepout circle 3 mm across on both layers) between them
+HOLE = """(pcb test.dsn
+  (resolution um 10)
+  (unit um)
+  (structure
+    (layer F.Cu (type signal))
+    (layer B.Cu (type signal))
+    (boundary (path pcb 0  0 0  20000 0  20000 10000  0 10000  0 0))
+    (via V)
+    (rule (width 200) (clearance 200))
+  )
+  (placement
+    (component P (place A1 2000 5000 front 0) (place B1 18000 5000 front 0))
+    (component M (place H1 10000 5000 front 0))
+  )
+  (library
+    (image P (pin TH 1 0 0))
+    (image M (keepout "" (circle F.Cu 3000)) (keepout "" (circle B.Cu 3000)))
+    (padstack TH (shape (circle F.Cu 800)) (shape (circle B.Cu 800)) (attach off))
+    (padstack V (shape (circle F.Cu 600)) (shape (circle B.Cu 600)) (attach off))
+  )
+  (network
+    (net N1 (pins A1-1 B1-1))
+    (class default N1 (circuit (use_via V)) (rule (width 200) (clearance 200)))
+  )
+)
+"""
+
+
+def test_route_goes_around_an_image_keepout(lib, tmp_path, ref):
+    """A part's image keepout (a hole) is placed with the part and obeyed: the straight line between the pins crosses
+    it, the route goes around, and the reference's judge sees no keepout violation."""
+    d = _write(tmp_path, "hole.dsn", HOLE)
+    b = R.Board.from_dsn(d, cell_um=50, lib_path=lib)
+    r = R.Router(b, R.default_options(R.library(lib)), lib_path=lib)
+    res = r.route()
+    assert res["opens"] == 0 and res["violations"] == 0
+    rows = r.net_route(0)
+    rows = rows[rows[:, 0] >= 0]
+    near = np.hypot(rows[:, 1] + 0.5 - 200, rows[:, 2] + 0.5 - 100)  # cells from the hole's centre
+    assert near.min() >= 30 + 2  # the hole's radius (1500 um) plus half the track width
+    s = str(tmp_path / "hole.ses")
+    r.write_ses(s)
+    (nets, conns, opens, viol), log = _ref_eval(ref, d, s)
+    assert (nets, conns, opens, viol) == (1, 1, 0, 0), log
