@@ -81,6 +81,8 @@ typedef struct pcbr_options {
                                  routed against the board as the batch began and all committed, overlaps left to the
                                  next round (parallel PathFinder); 2: also in the first round. Legalizing is always
                                  one net at a time. */
+    int32_t rescue;           /* while legalizing, a connection the net's window cannot hold is searched again over
+                                 the whole board (1, the default) or left open (0) */
 } pcbr_options;
 void pcbr_default_options(pcbr_options* o);
 
