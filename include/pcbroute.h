@@ -79,8 +79,9 @@ typedef struct pcbr_options {
     int32_t negotiate;        /* 0: waves commit in order with validation (the routes equal one net at a time);
                                  1: from the second negotiation round on, the nets of a batch (|wave| at once) are
                                  routed against the board as the batch began and all committed, overlaps left to the
-                                 next round (parallel PathFinder); 2: also in the first round. Legalizing is always
-                                 one net at a time. */
+                                 next round (parallel PathFinder); 2: also in the first round; 3: as 1 until two
+                                 rounds in a row bring no 10 % fewer conflicts, then in order as 0 (with a fresh
+                                 patience). Legalizing is always one net at a time. */
     int32_t rescue;           /* while legalizing, a connection the net's window cannot hold is searched again over
                                  the whole board (1, the default) or left open (0) */
 } pcbr_options;

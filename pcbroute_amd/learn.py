@@ -82,14 +82,21 @@ def corridor_stats(net, data, thr=0.05):
     return recall, kept
 
 
-CONFIGS = {  # name: (wave, group_mm, prior); every configuration's routes equal routing its order one net at a time
-    "plain": (0, 1.0, False),
-    "learned": (0, 1.0, True),
-    "batched_learned": (16, 1.0, True),    # speculative pipeline, commits validated in order
-    "grouped_learned": (-64, 1.0, True),   # box-disjoint groups routed together, commits validated in order
+CONFIGS = {  # name: (wave, group_mm, prior, negotiate)
+    # these four route as their order routed one net at a time
+    "plain": (0, 1.0, False, 0),
+    "learned": (0, 1.0, True, 0),
+    "batched_learned": (16, 1.0, True, 0),    # speculative pipeline, commits validated in order
+    "grouped_learned": (-64, 1.0, True, 0),   # box-disjoint groups routed together, commits validated in order
+    # parallel PathFinder: batches of 64 nets routed against the board as the batch began, all committed
+    "parallel_learned": (64, 1.0, True, 1),   # from the second negotiation round on
+    "parallel_learned_all": (64, 1.0, True, 2),  # every round
+    # parallel rounds while they cut conflicts, then box-disjoint groups committed in order
+    "hybrid_learned": (-64, 1.0, True, 3),
+    "hybrid_plain": (-64, 1.0, False, 3),
 }
-FIELDS = ("opens", "violations", "route_s", "search_s", "prior_s", "searches", "rounds", "revalidated", "iterations",
-          "search_fallbacks", "length_cells", "vias")
+FIELDS = ("opens", "violations", "route_s", "search_s", "prior_s", "upload_s", "backtrace_s", "host_s", "searches",
+          "rounds", "revalidated", "iterations", "search_fallbacks", "length_cells", "vias")
 
 
 REF = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "pcbroute")
@@ -122,8 +129,8 @@ def evaluate(net, seeds, configs=CONFIGS, reference=False):
             dsn = os.path.join(tmp, "board.dsn")
             if reference:
                 b.write_dsn(dsn)
-            for name, (wave, group, use) in configs.items():
-                r = R.Router(b, R.default_options(wave=wave, group_mm=group))
+            for name, (wave, group, use, neg) in configs.items():
+                r = R.Router(b, R.default_options(wave=wave, group_mm=group, negotiate=neg))
                 if use:
                     r.set_prior(net)
                 res = r.route()
@@ -184,6 +191,8 @@ def main():
     ap.add_argument("--reference", action="store_true",
                     help="judge every session with the reference's --eval and route each board with the reference's "
                          "CPU router too (needs oracle/_ref/pcbroute)")
+    ap.add_argument("--configs", default=",".join(CONFIGS),
+                    help="comma-separated configurations to evaluate (plain always included)")
     a = ap.parse_args()
     t0 = time.time()
     rep = {"crop": CROP}
@@ -210,7 +219,12 @@ def main():
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         np.save(a.out, net.get_params())
         rep["collect_boards"] = figs
-    rows = evaluate(net, [EVAL_SEED0 + k for k in range(a.eval)], reference=a.reference and os.path.exists(REF))
+    names = ["plain"] + [c for c in a.configs.split(",") if c and c != "plain"]
+    bad = [c for c in names if c not in CONFIGS]
+    if bad:
+        raise SystemExit(f"unknown configurations: {', '.join(bad)} (known: {', '.join(CONFIGS)})")
+    rows = evaluate(net, [EVAL_SEED0 + k for k in range(a.eval)], configs={c: CONFIGS[c] for c in names},
+                    reference=a.reference and os.path.exists(REF))
     rep["eval"] = rows
     rep["summary"] = summary(rows)
     rep["total_s"] = time.time() - t0
