@@ -84,6 +84,8 @@ typedef struct pcbr_options {
                                  patience). Legalizing is always one net at a time. */
     int32_t rescue;           /* while legalizing, a connection the net's window cannot hold is searched again over
                                  the whole board (1, the default) or left open (0) */
+    int32_t smooth;           /* pcbr_write_ses smooths the tracks (1, the default, as src/main.cpp:287-290) or writes
+                                 the grid paths' straight runs (0) */
 } pcbr_options;
 void pcbr_default_options(pcbr_options* o);
 
@@ -128,9 +130,12 @@ int pcbr_set_prior(pcbr_router* r, pcbprior* net);
  * src/router.cpp:660-687 (outside a net's map 0, a layer the file lacks no say); nets without a map route plainly.
  * Returns the number of the board's nets the file has a map for, or -1 (bad magic, pitch, sizes, a short file). */
 int pcbr_write_prior(const pcbr_router* r, const char* path);
-/* The routes as a Specctra session file (write_ses, src/router.cpp:2938-2963): per net its wires from cell centre to
- * cell centre at its class's width, collinear steps merged, and a via of its class at every layer change; the
- * board's names and coordinates (a generated board: nets N<k>, layers L<l>, vias V<class>). 0 or -1. */
+/* The routes as a Specctra session file (write_ses, src/router.cpp:2938-2963): per net its wires at its class's width
+ * and a via of its class at every layer change; the board's names and coordinates (a generated board: nets N<k>,
+ * layers L<l>, vias V<class>). With the smooth option (smooth_routes, src/smooth.cpp:105-225) each chain of track
+ * between fixed vertices (vias, the net's pads, ends, junctions) is redrawn with the fewest 0/45/90-degree segments
+ * that clear other nets' copper, keepouts and the board edge exactly, nets one after another; otherwise the wires are
+ * the grid paths' straight runs from cell centre to cell centre. 0 or -1. */
 int pcbr_write_ses(const pcbr_router* r, const char* path);
 int pcbr_read_prior(pcbr_router* r, const char* path);
 /* Training samples: when enabled, the first route of every net in the first round records its input features
