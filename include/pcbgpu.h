@@ -126,6 +126,10 @@ int pcbgpu_grid_upload(pcbgpu_grid* g, int32_t l, int32_t j0, int32_t nrows, con
 /* the same for the cells [i0, i0 + w) x [j0, j0 + h) only (a commit's footprint) */
 int pcbgpu_grid_upload_rect(pcbgpu_grid* g, int32_t l, int32_t i0, int32_t j0, int32_t w, int32_t h, const int32_t* cst,
                             const uint16_t* st);
+/* several such rectangles at once, one staged copy and one wait (a commit's footprint on every layer and the via
+ * grid): rectangle k is {l, i0, j0, w, h} = rects[5k .. 5k + 4], read from cst[k] / st[k]; 0 or -1 (nothing copied) */
+int pcbgpu_grid_upload_rects(pcbgpu_grid* g, int32_t n, const int32_t* rects, const int32_t* const* cst,
+                             const uint16_t* const* st);
 /* a solve whose cst/st/vcst/vst come from the grid (the query's pointers for them are ignored) */
 int pcbgpu_solve_grid(const pcbgpu_grid* g, const pcbgpu_query* q, pcbgpu_out* out, int wait);
 /* one rectangle of the window, no widening (src/gpu.cu:422-546 solve_rect). flags: PCBGPU_NOFIELD leaves the

@@ -79,6 +79,15 @@ int pcbgpu_grid_upload_rect(pcbgpu_grid* g, int32_t l, int32_t i0, int32_t j0, i
     return 0;
 }
 
+int pcbgpu_grid_upload_rects(pcbgpu_grid* g, int32_t n, const int32_t* rects, const int32_t* const* cst,
+                             const uint16_t* const* st) {
+    for (int k = 0; k < n; ++k)
+        if (pcbgpu_grid_upload_rect(g, rects[5 * k], rects[5 * k + 1], rects[5 * k + 2], rects[5 * k + 3],
+                                    rects[5 * k + 4], cst[k], st[k]) != 0)
+            return -1;
+    return 0;
+}
+
 int pcbgpu_solve_grid(const pcbgpu_grid* g, const pcbgpu_query* q, pcbgpu_out* out, int) {
     const pcbgpu_query x = on_grid(g, q);
     return orc_solve(&x, out, size_t(1) << 31);
