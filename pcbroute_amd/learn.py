@@ -82,18 +82,24 @@ def corridor_stats(net, data, thr=0.05):
     return recall, kept
 
 
-CONFIGS = {  # name: (wave, group_mm, prior, negotiate)
+CONFIGS = {  # name: router options, and "prior": route with the network's prior
     # these four route as their order routed one net at a time
-    "plain": (0, 1.0, False, 0),
-    "learned": (0, 1.0, True, 0),
-    "batched_learned": (16, 1.0, True, 0),    # speculative pipeline, commits validated in order
-    "grouped_learned": (-64, 1.0, True, 0),   # box-disjoint groups routed together, commits validated in order
+    "plain": dict(wave=0),
+    "learned": dict(wave=0, prior=True),
+    "batched_learned": dict(wave=16, prior=True),    # speculative pipeline, commits validated in order
+    "grouped_learned": dict(wave=-64, prior=True),   # box-disjoint groups routed together, commits validated in order
     # parallel PathFinder: batches of 64 nets routed against the board as the batch began, all committed
-    "parallel_learned": (64, 1.0, True, 1),   # from the second negotiation round on
-    "parallel_learned_all": (64, 1.0, True, 2),  # every round
+    "parallel_learned": dict(wave=64, negotiate=1, prior=True),      # from the second negotiation round on
+    "parallel_learned_all": dict(wave=64, negotiate=2, prior=True),  # every round
     # parallel rounds while they cut conflicts, then box-disjoint groups committed in order
-    "hybrid_learned": (-64, 1.0, True, 3),
-    "hybrid_plain": (-64, 1.0, False, 3),
+    "hybrid_learned": dict(wave=-64, negotiate=3, prior=True),
+    "hybrid_plain": dict(wave=-64, negotiate=3),
+    # the same with less patience for the in-order tail (stop after this many rounds without a 10 % gain)
+    "hybrid_learned_p3": dict(wave=-64, negotiate=3, patience=3, prior=True),
+    "hybrid_learned_p2": dict(wave=-64, negotiate=3, patience=2, prior=True),
+    "grouped_learned_p3": dict(wave=-64, patience=3, prior=True),
+    "plain_p3": dict(wave=0, patience=3),
+    "parallel16_learned": dict(wave=16, negotiate=1, prior=True),
 }
 FIELDS = ("opens", "violations", "route_s", "search_s", "prior_s", "upload_s", "backtrace_s", "host_s", "searches",
           "rounds", "revalidated", "iterations", "search_fallbacks", "length_cells", "vias")
@@ -129,9 +135,10 @@ def evaluate(net, seeds, configs=CONFIGS, reference=False):
             dsn = os.path.join(tmp, "board.dsn")
             if reference:
                 b.write_dsn(dsn)
-            for name, (wave, group, use, neg) in configs.items():
-                r = R.Router(b, R.default_options(wave=wave, group_mm=group, negotiate=neg))
-                if use:
+            for name, cfg in configs.items():
+                opts = {k: v for k, v in cfg.items() if k != "prior"}
+                r = R.Router(b, R.default_options(**opts))
+                if cfg.get("prior"):
                     r.set_prior(net)
                 res = r.route()
                 row[name] = {k: res[k] for k in FIELDS}
