@@ -158,10 +158,10 @@ int pcbgpu_round_result(pcbgpu_round* r, int ticket, pcbgpu_out* out);
 void pcbgpu_round_end(pcbgpu_round* r);
 
 /* ---- tuning and diagnostics */
-/* tile rows per workgroup and local sweeps per tile visit; 0 keeps the current value. Returns 0 or -1. Until it is
- * called, routing windows (rectangles of at most 512 tiles of 64 x 16 cells, whose passes are latency-bound) run 32
- * rows and 2 local sweeps and larger rectangles the default (pcbgpu_get_tile); once called, its setting serves every
- * rectangle. Results never depend on it. */
+/* tile rows per workgroup (16, 32, -1: by layer count) and local sweeps per tile visit; 0 keeps the current value,
+ * (-1, -1) restores the defaults. Returns 0 or -1. By default routing windows (rectangles of at most 512 tiles of
+ * 64 x 16 cells, whose passes are latency-bound) run 32 rows and 2 local sweeps and larger rectangles rows by layer
+ * count and 4 sweeps (pcbgpu_get_tile); a setting serves every rectangle. Results never depend on it. */
 int pcbgpu_set_tile(int32_t tile_rows, int32_t local_sweeps);
 void pcbgpu_get_tile(int32_t nl, int32_t* tile_rows, int32_t* local_sweeps);
 /* delta-stepping band over tiles: a pass visits the queued tiles whose key (the least cost change that queued them)

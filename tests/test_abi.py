@@ -91,3 +91,22 @@ def test_tune_table_set_save_load(tmp_path):
             b.tune_get(9)
     finally:
         b.tune_set(4, None)
+
+
+def test_tile_setting_and_defaults():
+    """A tile setting (host only) reads back, (-1, -1) restores the defaults, bad settings are refused."""
+    from pcbroute_amd import Backend
+    b = Backend()
+    default = b.get_tile(4)
+    try:
+        b.set_tile(32, 3)
+        assert b.get_tile(4) == (32, 3)
+        b.set_tile(0, 5)  # 0 keeps the rows
+        assert b.get_tile(4) == (32, 5)
+        with pytest.raises(Exception):
+            b.set_tile(24, 2)
+        with pytest.raises(Exception):
+            b.set_tile(16, -3)
+    finally:
+        b.set_tile(-1, -1)
+    assert b.get_tile(4) == default and default[1] == 4

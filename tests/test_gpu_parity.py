@@ -52,7 +52,7 @@ def test_tile_shapes_do_not_change_results(backend, oracle, rows, lsweeps):
     try:
         res = backend.solve(p)
     finally:
-        backend.set_tile(-1, 8)
+        backend.set_tile(-1, -1)  # the defaults again
     _same(res, ref)
 
 
