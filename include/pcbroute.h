@@ -86,6 +86,9 @@ typedef struct pcbr_options {
                                  the whole board (1, the default) or left open (0) */
     int32_t smooth;           /* pcbr_write_ses smooths the tracks (1, the default, as src/main.cpp:287-290) or writes
                                  the grid paths' straight runs (0) */
+    int32_t ml_rounds;        /* the learned prior (network or file) guides a net's searches in the first negotiation
+                                 round only (0, the default) or in every round (1, as src/router.cpp:674,1218 keep
+                                 a net's map for all its searches); never while legalizing */
 } pcbr_options;
 void pcbr_default_options(pcbr_options* o);
 

@@ -99,6 +99,10 @@ CONFIGS = {  # name: router options, and "prior": route with the network's prior
     "hybrid_learned_p2": dict(wave=-64, negotiate=3, patience=2, prior=True),
     "grouped_learned_p3": dict(wave=-64, patience=3, prior=True),
     "plain_p3": dict(wave=0, patience=3),
+    # the prior in every negotiation round, as the reference applies a net's map to all its searches
+    "learned_all": dict(wave=0, ml_rounds=1, prior=True),
+    "grouped_learned_all": dict(wave=-64, ml_rounds=1, prior=True),
+    "hybrid_learned_all": dict(wave=-64, negotiate=3, ml_rounds=1, prior=True),
     "parallel16_learned": dict(wave=16, negotiate=1, prior=True),
 }
 FIELDS = ("opens", "violations", "route_s", "search_s", "prior_s", "upload_s", "backtrace_s", "host_s", "searches",

@@ -34,7 +34,8 @@ class Options(C.Structure):
                 ("via_cost_mm", C.c_double), ("wrong_way", C.c_double), ("wave", C.c_int32), ("group_mm", C.c_double),
                 ("ml_weight", C.c_double), ("ml_sat", C.c_double), ("ml_corridor", C.c_double),
                 ("ml_margin_mm", C.c_double), ("legalize", C.c_int32), ("verbose", C.c_int32),
-                ("negotiate", C.c_int32), ("rescue", C.c_int32), ("smooth", C.c_int32)]
+                ("negotiate", C.c_int32), ("rescue", C.c_int32), ("smooth", C.c_int32),
+                ("ml_rounds", C.c_int32)]
 
 
 class Result(C.Structure):
