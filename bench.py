@@ -33,7 +33,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(board, pairs_seed, n_sample, threads):
+def cpu_baseline(board, pairs_seed, n_sample, threads, n_total=64):
     """The oracle's A* (the CPU search the back end finishes) on a bounded sample of the same searches, and one
     full-field Dijkstra (what the back end computes) on one thread."""
     from oracle import Oracle
@@ -48,7 +48,7 @@ def cpu_baseline(board, pairs_seed, n_sample, threads):
     dfull = time.perf_counter() - t1
     return dict(value=round(n_sample / dt, 3), unit="searches/s", cores=threads, kind="port",
                 sample=f"A* source-to-target (the CPU search the back end finishes) on the first {n_sample} of the "
-                       f"64 searches, {threads} threads, {dt:.1f} s, {int(pops.sum())} pops; one full-field "
+                       f"{n_total} searches, {threads} threads, {dt:.1f} s, {int(pops.sum())} pops; one full-field "
                        f"Dijkstra of the same grid on 1 thread: {dfull:.2f} s",
                 full_field_s=round(dfull, 3))
 
@@ -315,8 +315,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=("both", "single", "round"), default="both")
-    ap.add_argument("--workload", choices=("kernel", "round", "prior", "c1"), default="kernel",
-                    help="kernel: the 64 whole-grid searches (the metric); round: 256 net searches of a speculative "
+    ap.add_argument("--workload", choices=("kernel", "round", "prior", "c1", "c5"), default="kernel",
+                    help="kernel: the 64 whole-grid searches (the metric); c5: the same on a 3200x2400x8 grid (the "
+                         "C5 board's 61.4M states), 16 searches; round: 256 net searches of a speculative "
                          "round on a 2400x1800x6 board with two net classes, as one round and one at a time; "
                          "prior: the corridor CNN's training and inference on full 4-layer boards; c1: the router on "
                          "the C1 board, its session judged by the reference's evaluator, the reference's CPU router "
@@ -361,7 +362,10 @@ def main():
     if a.workload == "c1":
         return c1_workload(b, a)
     log("building the board")
-    board = synth.kernel_board(a.board_seed)
+    dims = dict(nx=3200, ny=2400, nl=8) if a.workload == "c5" else {}
+    if a.workload == "c5" and a.pairs == 64:
+        a.pairs = 16
+    board = synth.kernel_board(a.board_seed, **dims)
     probs = synth.pair_problems(board, a.pairs_seed, a.pairs)
     nl, ny, nx = board[0].shape
     g = b.grid(probs[0])
@@ -426,11 +430,12 @@ def main():
             line["roofline"]["traffic_source"] = t["source"]
     # the same round with every search's canonical field copied back into pinned host memory (what a caller that
     # backtraces on the host pays): reported beside the value, never as it
-    g2 = b.grid(probs[0])
-    t_fields = round_with_fields(b, g2, probs)
-    line["with_fields"] = {"ms_per_search": round(t_fields / len(probs) * 1e3, 3),
-                           "field_mb_per_search": round(cells * 4 / 1e6, 1)}
-    g2.close()
+    if a.workload == "kernel":
+        g2 = b.grid(probs[0])
+        t_fields = round_with_fields(b, g2, probs)
+        line["with_fields"] = {"ms_per_search": round(t_fields / len(probs) * 1e3, 3),
+                               "field_mb_per_search": round(cells * 4 / 1e6, 1)}
+        g2.close()
     if "single" in res and head != "single":
         w1, s1, sw1 = res["single"]
         line["one_at_a_time"] = {"ms_per_search": round(w1 / n_search * 1e3, 3),
@@ -438,7 +443,7 @@ def main():
                                  "passes_per_search": round(sum(sw1) / len(sw1), 2)}
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        cb = cpu_baseline(board, a.pairs_seed, min(a.cpu_sample, a.pairs), threads)
+        cb = cpu_baseline(board, a.pairs_seed, min(a.cpu_sample, a.pairs), threads, a.pairs)
         line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 

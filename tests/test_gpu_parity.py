@@ -149,6 +149,31 @@ def test_full_size_kernel_board(backend, oracle, k):
     np.testing.assert_array_equal(res.d, d)
 
 
+@pytest.mark.parametrize("nl", [6, 7, 8])
+def test_many_layers(backend, oracle, nl):
+    """Six to eight layers (the C3 and C5 boards' counts), both kernel variants."""
+    for k, sym in enumerate((False, True)):
+        p = random_case(1200 + 10 * nl + k, nx=130, ny=97, nl=nl, sym=sym)
+        _say("layers", nl, "sym", sym, p.window)
+        _same(backend.solve(p), oracle.solve(p))
+
+
+def test_c5_size_search(backend, oracle):
+    """One search over a whole 3200 x 2400 x 8 grid (61.4M states, the size of the C5 board) against the oracle's
+    Dijkstra, bit for bit."""
+    board = synth.kernel_board(5, nx=3200, ny=2400, nl=8)
+    p = synth.pair_problems(board, 11, 1)[0]
+    _say("c5 size")
+    g = backend.grid(p)
+    try:
+        res = backend.solve_rect_grid(g, p)
+    finally:
+        g.close()
+    d, best, opt = oracle.solve_rect(p)
+    assert res.best == best
+    np.testing.assert_array_equal(res.d, d)
+
+
 @pytest.mark.parametrize("nl", [2, 4])
 def test_round_equals_one_at_a_time(backend, oracle, nl):
     """A round of searches with different windows and two grids (two net classes) gives each search exactly its
