@@ -104,6 +104,10 @@ CONFIGS = {  # name: router options, and "prior": route with the network's prior
     "grouped_learned_all": dict(wave=-64, ml_rounds=1, prior=True),
     "hybrid_learned_all": dict(wave=-64, negotiate=3, ml_rounds=1, prior=True),
     "parallel16_learned": dict(wave=16, negotiate=1, prior=True),
+    # parallel rounds with more patience and faster-growing congestion costs (Jacobi PathFinder needs more rounds)
+    "parallel_learned_slow": dict(wave=64, negotiate=1, patience=12, prior=True),
+    "parallel_learned_hist": dict(wave=64, negotiate=1, patience=12, hist_inc=1.0, pres_growth=2.0, prior=True),
+    "parallel16_learned_hist": dict(wave=16, negotiate=1, patience=12, hist_inc=1.0, pres_growth=2.0, prior=True),
 }
 FIELDS = ("opens", "violations", "route_s", "search_s", "prior_s", "upload_s", "backtrace_s", "host_s", "searches",
           "rounds", "revalidated", "iterations", "search_fallbacks", "length_cells", "vias")
