@@ -20,7 +20,7 @@ EXPORTS = (
     "pcbgpu_solve_rect_grid", "pcbgpu_round_begin", "pcbgpu_round_submit", "pcbgpu_round_submit_rect", "pcbgpu_round_flags", "pcbgpu_round_launch",
     "pcbgpu_round_result", "pcbgpu_round_end", "pcbgpu_set_tile", "pcbgpu_get_tile", "pcbgpu_set_delta", "pcbgpu_stats_enable",
     "pcbgpu_stats_read", "pcbgpu_last_error", "pcbgpu_autotune", "pcbgpu_tune_get", "pcbgpu_tune_set",
-    "pcbgpu_tune_save", "pcbgpu_tune_load",
+    "pcbgpu_tune_save", "pcbgpu_tune_load", "pcbgpu_grid_upload_rects",
 )
 
 
@@ -56,6 +56,9 @@ def load_library(path=LIB_PATH):
     lib.pcbgpu_grid_create.argtypes = [C.c_int32, C.c_int32, C.c_int32]
     lib.pcbgpu_grid_create.restype = C.c_void_p
     lib.pcbgpu_grid_destroy.argtypes = [C.c_void_p]
+    lib.pcbgpu_grid_upload_rects.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
+                                             C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.POINTER(C.c_uint16))]
+    lib.pcbgpu_grid_upload_rects.restype = C.c_int
     lib.pcbgpu_grid_upload.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                        C.POINTER(C.c_uint16)]
     lib.pcbgpu_grid_upload.restype = C.c_int
@@ -146,6 +149,24 @@ class Grid:
             if backend.lib.pcbgpu_grid_upload(self.h, l, 0, self.ny, c.ctypes.data_as(C.POINTER(C.c_int32)),
                                               s.ctypes.data_as(C.POINTER(C.c_uint16))) != 0:
                 raise BackendError(backend.error())
+
+    def upload_rects(self, rects, cst, st, vcst, vst):
+        """Rectangles (layer, i0, j0, w, h) of new maps (layer nl: the via grid) in one call
+        (pcbgpu_grid_upload_rects)."""
+        cst = np.ascontiguousarray(cst, np.int32)
+        st = np.ascontiguousarray(st, np.uint16)
+        vcst = np.ascontiguousarray(vcst, np.int32)
+        vst = np.ascontiguousarray(vst, np.uint16)
+        n = len(rects)
+        flat = np.ascontiguousarray(np.asarray(rects, np.int32).reshape(n * 5))
+        cs = (C.POINTER(C.c_int32) * max(1, n))()
+        ss = (C.POINTER(C.c_uint16) * max(1, n))()
+        for k, r in enumerate(rects):
+            c, s = (cst[r[0]], st[r[0]]) if r[0] < self.nl else (vcst, vst)
+            cs[k] = c.ctypes.data_as(C.POINTER(C.c_int32))
+            ss[k] = s.ctypes.data_as(C.POINTER(C.c_uint16))
+        if self.b.lib.pcbgpu_grid_upload_rects(self.h, n, flat.ctypes.data_as(C.POINTER(C.c_int32)), cs, ss) != 0:
+            raise BackendError(self.b.error())
 
     def close(self):
         if self.h:

@@ -149,6 +149,52 @@ def test_full_size_kernel_board(backend, oracle, k):
     np.testing.assert_array_equal(res.d, d)
 
 
+def test_grid_rect_uploads_batched(backend, oracle):
+    """Rectangles of new maps uploaded in one call (pcbgpu_grid_upload_rects, a commit's footprint on several layers
+    and the via grid) leave the grid holding the merged maps: its solves equal those of the merged maps uploaded
+    whole, and the oracle's."""
+    a = random_case(1300, nx=120, ny=90, nl=3)
+    b = random_case(1301, nx=120, ny=90, nl=3)
+    rects = [(0, 10, 5, 40, 30), (2, 61, 40, 50, 45), (3, 0, 0, 120, 7), (1, 119, 89, 1, 1)]  # layer 3: the vias
+    cst, st, vcst, vst = a.cst.copy(), a.st.copy(), a.vcst.copy(), a.vst.copy()
+    for l, i0, j0, w, h in rects:
+        if l < 3:
+            cst[l, j0:j0 + h, i0:i0 + w] = b.cst[l, j0:j0 + h, i0:i0 + w]
+            st[l, j0:j0 + h, i0:i0 + w] = b.st[l, j0:j0 + h, i0:i0 + w]
+        else:
+            vcst[j0:j0 + h, i0:i0 + w] = b.vcst[j0:j0 + h, i0:i0 + w]
+            vst[j0:j0 + h, i0:i0 + w] = b.vst[j0:j0 + h, i0:i0 + w]
+    from pcbroute_amd.abi import Problem
+    merged = Problem(cst, st, vcst, vst, starts=a.starts, targets=a.targets, window=a.window, me=a.me,
+                     via_cost=a.via_cost, dirw=a.dirw, lay=a.lay[:a.nl], own_cf=a.own_cf, own_vcf=a.own_vcf)
+    g, g2 = backend.grid(a), backend.grid(merged)
+    try:
+        g.upload_rects(rects, b.cst, b.st, b.vcst, b.vst)
+        _say("batched rect uploads", len(rects))
+        got, want = backend.solve_grid(g, merged), backend.solve_grid(g2, merged)
+    finally:
+        g.close()
+        g2.close()
+    assert got.rect == want.rect and got.best == want.best
+    np.testing.assert_array_equal(got.d, want.d)
+    _same(got, oracle.solve(merged))
+
+
+def test_window_tiles_equal_the_throughput_setting(backend, oracle):
+    """A routing window solved with the window default (32 rows, 2 local sweeps) and with an explicit 16-row,
+    4-sweep setting: the same rectangle, best state and field."""
+    p = random_case(1400, nx=300, ny=220, nl=4, sym=True)
+    auto = backend.solve(p)
+    backend.set_tile(16, 4)
+    try:
+        fixed = backend.solve(p)
+    finally:
+        backend.set_tile(-1, -1)
+    assert auto.rect == fixed.rect and auto.best == fixed.best
+    np.testing.assert_array_equal(auto.d, fixed.d)
+    _same(auto, oracle.solve(p))
+
+
 @pytest.mark.parametrize("nl", [6, 7, 8])
 def test_many_layers(backend, oracle, nl):
     """Six to eight layers (the C3 and C5 boards' counts), both kernel variants."""
