@@ -101,19 +101,40 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* x, const uint16_t*
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int n = blockIdx.z, y0 = blockIdx.y * CR, x0 = blockIdx.x * PX;
     const size_t img = size_t(n) * H * W;
-    for (int e = tid; e < C * KK / 8; e += 256) {  // weights: 8 BF16 per load
-        const int co = e / (KK / 8), q = e % (KK / 8);
-        const uint4 v = *reinterpret_cast<const uint4*>(wt + size_t(co) * KK + q * 8);
-        *reinterpret_cast<uint4*>(WS + co * WSP + q * 8) = v;
+    // Staging: every thread issues all its loads (weights: 8 BF16 each; the input patch, zero outside the image)
+    // before its first LDS store, so the block waits for one round trip, not one per load.
+    constexpr int NW = C * KK / 8, NX = (CR + 2) * (PX + 2) * (C / 8);
+    constexpr int WIT = (NW + 255) / 256, XIT = (NX + 255) / 256;
+    uint4 wv[WIT], xv[XIT];
+#pragma unroll
+    for (int it = 0; it < WIT; ++it) {
+        const int e = tid + it * 256;
+        wv[it] = make_uint4(0, 0, 0, 0);
+        if (e < NW) wv[it] = *reinterpret_cast<const uint4*>(wt + size_t(e / (KK / 8)) * KK + (e % (KK / 8)) * 8);
     }
-    for (int e = tid; e < (CR + 2) * (PX + 2) * (C / 8); e += 256) {  // the input patch, zero outside the image
+#pragma unroll
+    for (int it = 0; it < XIT; ++it) {
+        const int e = tid + it * 256;
         const int r = e / ((PX + 2) * (C / 8)), rem = e % ((PX + 2) * (C / 8)), q = rem / (C / 8), c8 = rem % (C / 8);
         const int py = y0 - 1 + r, px = x0 - 1 + q;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (py >= 0 && py < H && px >= 0 && px < W) v = *reinterpret_cast<const uint4*>(x + (img + size_t(py) * W + px) * C + c8 * 8);
-        uint2* d = reinterpret_cast<uint2*>(XS + (r * (PX + 2) + q) * XSP + c8 * 8);
-        d[0] = make_uint2(v.x, v.y);
-        d[1] = make_uint2(v.z, v.w);
+        xv[it] = make_uint4(0, 0, 0, 0);
+        if (e < NX && py >= 0 && py < H && px >= 0 && px < W)
+            xv[it] = *reinterpret_cast<const uint4*>(x + (img + size_t(py) * W + px) * C + c8 * 8);
+    }
+#pragma unroll
+    for (int it = 0; it < WIT; ++it) {
+        const int e = tid + it * 256;
+        if (e < NW) *reinterpret_cast<uint4*>(WS + (e / (KK / 8)) * WSP + (e % (KK / 8)) * 8) = wv[it];
+    }
+#pragma unroll
+    for (int it = 0; it < XIT; ++it) {
+        const int e = tid + it * 256;
+        if (e < NX) {
+            const int r = e / ((PX + 2) * (C / 8)), rem = e % ((PX + 2) * (C / 8)), q = rem / (C / 8), c8 = rem % (C / 8);
+            uint2* d = reinterpret_cast<uint2*>(XS + (r * (PX + 2) + q) * XSP + c8 * 8);
+            d[0] = make_uint2(xv[it].x, xv[it].y);
+            d[1] = make_uint2(xv[it].z, xv[it].w);
+        }
     }
     __syncthreads();
     const int pr = lane & 15, kq = lane >> 4;
