@@ -262,26 +262,40 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* x, const uint16_t
                                 : make_uint4(0, 0, 0, 0);
             put4(DT + (c8 * 8) * DTP + r * PX + 4 * g, DTP, v);
         }
-        for (int e = tid; e < (WR + 2) * (XTW / 4) * (C / 8); e += 256) {  // input rows c0 - 1 .. c0 + WR
-            const int r = e / ((XTW / 4) * (C / 8)), rem = e % ((XTW / 4) * (C / 8)), g = rem / (C / 8), c8 = rem % (C / 8);
-            const int py = c0 - 1 + r;
-            uint4 v[4];
+        {  // input rows c0 - 1 .. c0 + WR: every load of the thread issued before its first LDS store
+            constexpr int NXT = (WR + 2) * (XTW / 4) * (C / 8), XIT = (NXT + 255) / 256;
+            uint4 v[XIT][4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int px = xb - 4 + 4 * g + j;
-                v[j] = (py >= 0 && py < H && px >= 0 && px < W)
-                           ? *reinterpret_cast<const uint4*>(x + (img + size_t(py) * W + px) * C + c8 * 8)
-                           : make_uint4(0, 0, 0, 0);
+            for (int it = 0; it < XIT; ++it) {
+                const int e = tid + it * 256;
+                const int r = e / ((XTW / 4) * (C / 8)), rem = e % ((XTW / 4) * (C / 8)), g = rem / (C / 8), c8 = rem % (C / 8);
+                const int py = c0 - 1 + r;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int px = xb - 4 + 4 * g + j;
+                    v[it][j] = (e < NXT && py >= 0 && py < H && px >= 0 && px < W)
+                                   ? *reinterpret_cast<const uint4*>(x + (img + size_t(py) * W + px) * C + c8 * 8)
+                                   : make_uint4(0, 0, 0, 0);
+                }
             }
-            put4(XT + (c8 * 8) * XTP + r * XTW + 4 * g, XTP, v);
+#pragma unroll
+            for (int it = 0; it < XIT; ++it) {
+                const int e = tid + it * 256;
+                const int r = e / ((XTW / 4) * (C / 8)), rem = e % ((XTW / 4) * (C / 8)), g = rem / (C / 8), c8 = rem % (C / 8);
+                if (e < NXT) put4(XT + (c8 * 8) * XTP + r * XTW + 4 * g, XTP, v[it]);
+            }
         }
         __syncthreads();
-        if (tid < C)
-            for (int q = 0; q < rows * PX; q += 4) {
-                const v4s d4 = ld_px4(DT + tid * DTP, q, 0);
+        {  // the bias gradient: thread t sums 32 pixels of channel t % 32 (the 8 partial sums meet at the end)
+            const int ch = tid & (C - 1), q0 = (tid / C) * 32;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) bsum += bf2f(uint16_t(d4[j]));
-            }
+            for (int q = q0; q < q0 + 32; q += 4)
+                if (q < rows * PX) {
+                    const v4s d4 = ld_px4(DT + ch * DTP, q, 0);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) bsum += bf2f(uint16_t(d4[j]));
+                }
+        }
         for (int r = 0; r < rows; ++r)
 #pragma unroll
             for (int ps = 0; ps < PX / 16; ++ps) {
@@ -311,7 +325,15 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* x, const uint16_t
 #pragma unroll
                 for (int j = 0; j < 4; ++j) atomicAdd(&dw[size_t(16 * t + kq * 4 + j) * KK + kt * 16 + pr], acc[t][u][j]);
         }
-    if (tid < C) atomicAdd(&db[tid], bsum);
+    float* red = reinterpret_cast<float*>(DT);  // free after the last chunk's barrier
+    red[tid] = bsum;
+    __syncthreads();
+    if (tid < C) {
+        float t = 0.f;
+#pragma unroll
+        for (int k = 0; k < 256 / C; ++k) t += red[tid + C * k];
+        atomicAdd(&db[tid], t);
+    }
 }
 
 // binary cross-entropy of the logits over the masked pixels; dz = (sigmoid(z) - t) m / count into the BF16 slots
