@@ -98,6 +98,8 @@ CONFIGS = {  # name: router options, and "prior": route with the network's prior
     "hybrid_learned_p3": dict(wave=-64, negotiate=3, patience=3, prior=True),
     "hybrid_learned_p2": dict(wave=-64, negotiate=3, patience=2, prior=True),
     "grouped_learned_p3": dict(wave=-64, patience=3, prior=True),
+    "grouped_learned_p2": dict(wave=-64, patience=2, prior=True),
+    "hybrid_learned_p4": dict(wave=-64, negotiate=3, patience=4, prior=True),
     "plain_p3": dict(wave=0, patience=3),
     # the prior in every negotiation round, as the reference applies a net's map to all its searches
     "learned_all": dict(wave=0, ml_rounds=1, prior=True),
