@@ -78,7 +78,7 @@ __global__ void k_to_bf16(const float* x, uint16_t* out, size_t npix, int cin) {
     out[i] = c < cin ? f2bf(x[p * cin + c]) : uint16_t(0);
 }
 
-// y = conv3x3(x, wt) + bias, the implicit GEMM above. A block takes 64 pixels of CR consecutive rows, one row per
+// y = conv3x3(x, wt) + bias, the implicit GEMM above. A block takes 64 pixels of CR (8) consecutive rows, one row per
 // wavefront, all 32 output channels. It stages the (CR + 2) x 66-pixel input patch and the layer's weights in LDS
 // once; a wavefront then holds 4 x 2 accumulator tiles (64 pixels x 32 channels) and per K step reads 4 A and 2 B
 // fragments from LDS for 8 matrix-core products. The results go through LDS so that each lane finishes one pixel:
@@ -86,16 +86,17 @@ __global__ void k_to_bf16(const float* x, uint16_t* out, size_t npix, int cin) {
 // coalesced BF16 stores of its 32 slots, or FP32 logits of the first cout channels. With `valid` (rows, columns per
 // image) the outputs outside image n's valid extent are 0, so an image padded into a larger batch is computed exactly
 // as alone at its valid size (whose conv reads zeros past its edges).
-constexpr int CR = 4;         // rows per block
+constexpr int CR = 8;         // rows per block
+constexpr int NT = 64 * CR;   // threads per block
 constexpr int XSP = 36;       // LDS pitch of a staged input pixel (BF16): 18 dwords, conflict-free 8-byte reads
 constexpr int WSP = KK + 8;   // LDS pitch of a staged weight row
 constexpr int OSP = C + 1;    // LDS pitch of an output pixel (FP32)
 constexpr int XS_N = (CR + 2) * (PX + 2) * XSP, WS_N = C * WSP;  // BF16 elements
-static_assert(CR * PX * OSP * 2 <= XS_N + WS_N, "the output tile reuses the staging space");
-__global__ __launch_bounds__(256) void k_conv(const uint16_t* x, const uint16_t* wt, const float* bias, uint16_t* ybf,
+constexpr int SM_N = XS_N + WS_N > CR * PX * OSP * 2 ? XS_N + WS_N : CR * PX * OSP * 2;  // staging, then output
+__global__ __launch_bounds__(NT) void k_conv(const uint16_t* x, const uint16_t* wt, const float* bias, uint16_t* ybf,
                                               float* yf, const uint16_t* mask, int relu, int cout, int H, int W,
                                               const int32_t* valid) {
-    __shared__ __align__(16) uint16_t sm[XS_N + WS_N];
+    __shared__ __align__(16) uint16_t sm[SM_N];
     uint16_t* XS = sm;
     uint16_t* WS = sm + XS_N;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -104,17 +105,17 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* x, const uint16_t*
     // Staging: every thread issues all its loads (weights: 8 BF16 each; the input patch, zero outside the image)
     // before its first LDS store, so the block waits for one round trip, not one per load.
     constexpr int NW = C * KK / 8, NX = (CR + 2) * (PX + 2) * (C / 8);
-    constexpr int WIT = (NW + 255) / 256, XIT = (NX + 255) / 256;
+    constexpr int WIT = (NW + NT - 1) / NT, XIT = (NX + NT - 1) / NT;
     uint4 wv[WIT], xv[XIT];
 #pragma unroll
     for (int it = 0; it < WIT; ++it) {
-        const int e = tid + it * 256;
+        const int e = tid + it * NT;
         wv[it] = make_uint4(0, 0, 0, 0);
         if (e < NW) wv[it] = *reinterpret_cast<const uint4*>(wt + size_t(e / (KK / 8)) * KK + (e % (KK / 8)) * 8);
     }
 #pragma unroll
     for (int it = 0; it < XIT; ++it) {
-        const int e = tid + it * 256;
+        const int e = tid + it * NT;
         const int r = e / ((PX + 2) * (C / 8)), rem = e % ((PX + 2) * (C / 8)), q = rem / (C / 8), c8 = rem % (C / 8);
         const int py = y0 - 1 + r, px = x0 - 1 + q;
         xv[it] = make_uint4(0, 0, 0, 0);
@@ -123,12 +124,12 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* x, const uint16_t*
     }
 #pragma unroll
     for (int it = 0; it < WIT; ++it) {
-        const int e = tid + it * 256;
+        const int e = tid + it * NT;
         if (e < NW) *reinterpret_cast<uint4*>(WS + (e / (KK / 8)) * WSP + (e % (KK / 8)) * 8) = wv[it];
     }
 #pragma unroll
     for (int it = 0; it < XIT; ++it) {
-        const int e = tid + it * 256;
+        const int e = tid + it * NT;
         if (e < NX) {
             const int r = e / ((PX + 2) * (C / 8)), rem = e % ((PX + 2) * (C / 8)), q = rem / (C / 8), c8 = rem % (C / 8);
             uint2* d = reinterpret_cast<uint2*>(XS + (r * (PX + 2) + q) * XSP + c8 * 8);
@@ -548,7 +549,7 @@ bool forward_dev(pcbprior* p, int n, int h, int w, const int32_t* valid = nullpt
     for (int L = 0; L < p->depth; ++L) {
         const bool last = L == p->depth - 1;
         Timed t(p, 0, 2.0 * np * p->lout[L] * 9 * p->lin[L]);
-        hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, p->s, p->act[L].p, p->wt.p + size_t(L) * C * KK,
+        hipLaunchKernelGGL(k_conv, grid, dim3(NT), 0, p->s, p->act[L].p, p->wt.p + size_t(L) * C * KK,
                            p->bias.p + L * C, last ? nullptr : p->act[L + 1].p, last ? p->logits.p : nullptr,
                            static_cast<const uint16_t*>(nullptr), last ? 0 : 1, p->cout, h, w, valid);
     }
@@ -592,7 +593,7 @@ bool backward_dev(pcbprior* p, int n, int h, int w) {
         if (L > 0) {
             // the gradient at layer L's input: the flipped-weight conv of dy, zero where that input's ReLU was off
             Timed t(p, 0, fl);
-            hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, p->s, dy, p->wf.p + size_t(L) * C * KK,
+            hipLaunchKernelGGL(k_conv, grid, dim3(NT), 0, p->s, dy, p->wf.p + size_t(L) * C * KK,
                                static_cast<const float*>(nullptr), dx, static_cast<float*>(nullptr), p->act[L].p, 0, 0,
                                h, w, static_cast<const int32_t*>(nullptr));
             std::swap(dy, dx);
