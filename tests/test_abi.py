@@ -110,3 +110,25 @@ def test_tile_setting_and_defaults():
     finally:
         b.set_tile(-1, -1)
     assert b.get_tile(4) == default and default[1] == 4
+
+
+def test_out_at_reads_the_solved_rectangle_only():
+    """pcbgpu_out_at (host only) maps a window state into the solved rectangle and answers INF outside it, also for
+    state indices past the last layer whose layer number would not fit 32 bits (GpuOut::operator(), src/gpu.h:39-50)."""
+    import numpy as np
+    lib = load_library()
+    INF = (1 << 30) - 1
+    nl, ww, wh, rx0, ry0, rw, rh = 2, 10, 7, 3, 2, 4, 3
+    d = np.arange(nl * rh * rw, dtype=np.int32)
+    o = Out(rx0=rx0, ry0=ry0, rw=rw, rh=rh, nl=nl, ww=ww, wh=wh, best=-1,
+            d=d.ctypes.data_as(C.POINTER(C.c_int32)), cap=d.nbytes)
+    for l in range(nl):
+        for j in range(wh):
+            for i in range(ww):
+                q = (l * wh + j) * ww + i
+                inside = rx0 <= i < rx0 + rw and ry0 <= j < ry0 + rh
+                want = int(d[(l * rh + j - ry0) * rw + i - rx0]) if inside else INF
+                assert lib.pcbgpu_out_at(C.byref(o), q) == want
+    WN = ww * wh
+    for q in (-1, nl * WN, (1 << 32) * WN + (ry0 * ww + rx0), (1 << 62)):
+        assert lib.pcbgpu_out_at(C.byref(o), q) == INF
