@@ -62,6 +62,9 @@ def load_library(path=LIB_PATH):
     lib.pcbgpu_grid_upload.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                        C.POINTER(C.c_uint16)]
     lib.pcbgpu_grid_upload.restype = C.c_int
+    lib.pcbgpu_grid_upload_rect.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                            C.POINTER(C.c_int32), C.POINTER(C.c_uint16)]
+    lib.pcbgpu_grid_upload_rect.restype = C.c_int
     lib.pcbgpu_solve_grid.argtypes = [C.c_void_p, C.POINTER(Query), C.POINTER(Out), C.c_int]
     lib.pcbgpu_solve_grid.restype = C.c_int
     lib.pcbgpu_solve_rect_grid.argtypes = [C.c_void_p, C.POINTER(Query), C.c_int32, C.c_int32, C.c_int32, C.c_int32,

@@ -180,6 +180,31 @@ def test_grid_rect_uploads_batched(backend, oracle):
     _same(got, oracle.solve(merged))
 
 
+def test_grid_uploads_outside_the_grid_are_refused(backend, oracle):
+    """Uploads whose rectangle leaves the grid are refused on the host before any copy, also where the corner's
+    coordinate sum passes 2^31 (it would wrap negative in 32 bits)."""
+    import ctypes as C
+    a = random_case(1302, nx=64, ny=40, nl=2)
+    g = backend.grid(a)
+    lib = backend.lib
+    c = np.zeros(16, np.int32)
+    s = np.zeros(16, np.uint16)
+    cp, sp = c.ctypes.data_as(C.POINTER(C.c_int32)), s.ctypes.data_as(C.POINTER(C.c_uint16))
+    big = 2**31 - 8
+    try:
+        for l, i0, j0, w, h in [(0, big, 0, 16, 1), (0, 0, big, 1, 16), (0, 60, 0, 5, 1), (3, 0, 0, 1, 1)]:
+            _say("refused rect", (l, i0, j0, w, h))
+            assert lib.pcbgpu_grid_upload_rect(g.h, l, i0, j0, w, h, cp, sp) == -1
+        assert lib.pcbgpu_grid_upload(g.h, 0, big, 16, cp, sp) == -1
+        rects = np.array([0, big, 0, 16, 1], np.int32)
+        cps, sps = (C.POINTER(C.c_int32) * 1)(cp), (C.POINTER(C.c_uint16) * 1)(sp)
+        assert lib.pcbgpu_grid_upload_rects(g.h, 1, rects.ctypes.data_as(C.POINTER(C.c_int32)), cps, sps) == -1
+        got = backend.solve_grid(g, a)
+    finally:
+        g.close()
+    _same(got, oracle.solve(a))
+
+
 def test_window_tiles_equal_the_throughput_setting(backend, oracle):
     """A routing window solved with the window default (32 rows, 2 local sweeps) and with an explicit 16-row,
     4-sweep setting: the same rectangle, best state and field."""
