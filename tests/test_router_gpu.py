@@ -142,26 +142,31 @@ def test_c1_session_clean_by_the_reference(tmp_path):
     assert conns == b.info.n_connections and opens == res["opens"] == 0 and viol == 0
 
 
-def test_gate_board_searches_on_the_gpu(tmp_path):
+
+@pytest.mark.parametrize("board", ["GATE", "HOLE", "FEATURES"])
+def test_dsn_boards_search_on_the_gpu(tmp_path, board):
     """The board of the reference's GPU gate test (tests/test_gpu_gate.py:18-41, 53-68: one net across a 20 x 10 mm,
-    2-layer board; with the GPU on, a positive count of searches finished on it and 0 unrouted): every search of
-    this router runs on the HIP back end, so the count of searches is that count; the net is routed, the independent
-    check is clean, and the session is clean by the reference's own evaluator where it is built."""
+    2-layer board; with the GPU on, a positive count of searches finished on it and 0 unrouted), of its keepout test
+    (tests/test_image_keepouts.py:19-46) and the feature board of the DSN tests, routed on the HIP back end: every
+    search of this router runs there, so the count of searches is the gate's count; every connection is made, the
+    independent check is clean, and the session is clean by the reference's own evaluator where it is built."""
     import re
     import subprocess
-    from test_dsn import GATE, REF
-    d, s = str(tmp_path / "gate.dsn"), str(tmp_path / "gate.ses")
+    import test_dsn
+    d, s = str(tmp_path / "b.dsn"), str(tmp_path / "b.ses")
     with open(d, "w") as f:
-        f.write(GATE)
+        f.write(getattr(test_dsn, board))
     b = R.Board.from_dsn(d, cell_um=50)
     r = R.Router(b)
     res = r.route()
-    _say("gate:", res["searches"], "searches on the GPU,", res["opens"], "open")
+    _say(board, res["searches"], "searches on the GPU,", res["opens"], "open")
     assert res["searches"] > 0 and res["search_fallbacks"] == 0
     assert res["opens"] == 0 and res["violations"] == 0 and r.eval() == (0, 0)
     r.write_ses(s)
-    if os.path.exists(REF):
-        out = subprocess.run([REF, "-de", d, "--eval", s], capture_output=True, text=True, timeout=120)
+    if os.path.exists(test_dsn.REF):
+        out = subprocess.run([test_dsn.REF, "-de", d, "--eval", s], capture_output=True, text=True, timeout=120)
         m = re.search(r"eval: (\d+) nets, (\d+) connections, (\d+) open, (\d+) violations", out.stdout)
         assert m, out.stdout + out.stderr
-        assert tuple(int(x) for x in m.groups()) == (1, 1, 0, 0), out.stdout
+        _say(board, "by the reference:", out.stdout.strip())
+        nets, conns, opens, viol = (int(x) for x in m.groups())
+        assert conns == b.info.n_connections and opens == 0 and viol == 0, out.stdout
