@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from pcbroute_amd import synth
-from pcbroute_amd.abi import INF
+from pcbroute_amd.abi import INF, Problem
 
 from cases import random_case
 
@@ -245,7 +245,7 @@ def test_c5_size_search(backend, oracle):
     np.testing.assert_array_equal(res.d, d)
 
 
-@pytest.mark.parametrize("nl", [2, 4])
+@pytest.mark.parametrize("nl", [2, 4, 6])
 def test_round_equals_one_at_a_time(backend, oracle, nl):
     """A round of searches with different windows and two grids (two net classes) gives each search exactly its
     one-at-a-time result."""
@@ -263,6 +263,36 @@ def test_round_equals_one_at_a_time(backend, oracle, nl):
             _same(r, ref)
     finally:
         for g in grids:
+            g.close()
+
+
+def test_c3_round_against_the_oracle(backend, oracle):
+    """The batched speculative round at the C3 board's shape, with fewer and smaller searches: a 2400 x 1800 x 6 board
+    seen by two net classes (clearances of 1 and 2 cells), 32 searches of 2-8 pins whose boxes are log-uniform in
+    64..512 cells plus a 40-cell margin, tuned by the autotuner on another draw and then run as one round (passes
+    replayed from captured hipGraph chunks). Every search's rectangle, best state and field equal the oracle's
+    Dijkstra with widening; the round's pass counts equal its lone solves'."""
+    grids = synth.class_grids(3)
+    searches = synth.net_searches(grids, 17, 32, smax=512)
+    sample = synth.net_searches(grids, 1017, 12, smax=512)
+    gs = [backend.grid(Problem(c, s, vc, vs, starts=[], targets=[])) for c, s, vc, vs in grids]
+    nl = grids[0][0].shape[0]
+    try:
+        _say("c3 autotune", len(sample))
+        t = backend.autotune([(gs[k], p) for k, p in sample], reps=1)
+        assert backend.get_tile(nl) == (t["tile_rows"], t["local_sweeps"])
+        pairs = [(gs[k], p) for k, p in searches]
+        _say("c3 round", len(pairs), "setting", t["tile_rows"], t["local_sweeps"], t["delta"])
+        got = backend.round(pairs)
+        assert sum(r.best >= 0 for r in got) >= len(pairs) // 2
+        for k, ((g, p), r) in enumerate(zip(pairs, got)):
+            _say("c3 search", k, p.window)
+            _same(r, oracle.solve(p))
+            if k < 4:
+                assert r.sweeps == backend.solve_grid(g, p).sweeps
+    finally:
+        backend.tune_set(nl, None)
+        for g in gs:
             g.close()
 
 
