@@ -138,6 +138,21 @@ int pcbgpu_solve_grid(const pcbgpu_grid* g, const pcbgpu_query* q, pcbgpu_out* o
 int pcbgpu_solve_rect_grid(const pcbgpu_grid* g, const pcbgpu_query* q, int32_t rx0, int32_t ry0, int32_t rw,
                            int32_t rh, pcbgpu_out* out, int flags);
 
+/* ---- a solve that allocates nothing (no counterpart in the reference). For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. pcbgpu_solve_grid's
+ * search and result, with every device buffer of the solve taken from one fixed scratch array per device
+ * (PCBGPU_NOALLOC_SCRATCH_BYTES, held from the library's load until the process ends), uploads made straight from
+ * the caller's arrays, and the canonical field written into the caller's `field` (field_cap values; ww * wh * nl
+ * always suffice; NULL: no field). out gets the rectangle, best, sweeps and tries; out->d and out->cap are not
+ * touched. A search whose buffers do not fit the scratch fails (PCBGPU_ERR_FAIL, "needs more memory than the no-alloc
+ * scratch holds"): about 12 bytes per rectangle state plus 4 per cell, i.e. up to ~5M states. Calls in one process
+ * take turns on a mutex; the scratch keeps the last call's contents. It does not take one of the slots, runs on the
+ * device's null stream, and calls no hipMalloc, hipHostMalloc or hipFree (host lists of a call use the C++ heap).
+ * Kernel accounting (pcbgpu_stats) does not see it. Returns sweeps >= 0 or PCBGPU_ERR_FAIL. */
+#define PCBGPU_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int pcbgpu_solve_grid_noalloc(const pcbgpu_grid* g, const pcbgpu_query* q, int32_t* field, size_t field_cap,
+                              pcbgpu_out* out);
+
 /* ---- rounds: the speculative searches of one PathFinder round (src/router.cpp:2119-2242) in one sequence of
  * launches. The searches are independent; each keeps its own field, bound and widening, and its result equals what
  * pcbgpu_solve_grid returns for it alone. Submitted queries (and the grids) are borrowed until

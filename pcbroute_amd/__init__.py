@@ -20,7 +20,7 @@ EXPORTS = (
     "pcbgpu_solve_rect_grid", "pcbgpu_round_begin", "pcbgpu_round_submit", "pcbgpu_round_submit_rect", "pcbgpu_round_flags", "pcbgpu_round_launch",
     "pcbgpu_round_result", "pcbgpu_round_end", "pcbgpu_set_tile", "pcbgpu_get_tile", "pcbgpu_set_delta", "pcbgpu_stats_enable",
     "pcbgpu_stats_read", "pcbgpu_last_error", "pcbgpu_autotune", "pcbgpu_tune_get", "pcbgpu_tune_set",
-    "pcbgpu_tune_save", "pcbgpu_tune_load", "pcbgpu_grid_upload_rects",
+    "pcbgpu_tune_save", "pcbgpu_tune_load", "pcbgpu_grid_upload_rects", "pcbgpu_solve_grid_noalloc",
 )
 
 
@@ -70,6 +70,9 @@ def load_library(path=LIB_PATH):
     lib.pcbgpu_solve_rect_grid.argtypes = [C.c_void_p, C.POINTER(Query), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                            C.POINTER(Out), C.c_int]
     lib.pcbgpu_solve_rect_grid.restype = C.c_int
+    lib.pcbgpu_solve_grid_noalloc.argtypes = [C.c_void_p, C.POINTER(Query), C.POINTER(C.c_int32), C.c_size_t,
+                                              C.POINTER(Out)]
+    lib.pcbgpu_solve_grid_noalloc.restype = C.c_int
     lib.pcbgpu_set_tile.argtypes = [C.c_int32, C.c_int32]
     lib.pcbgpu_set_tile.restype = C.c_int
     lib.pcbgpu_get_tile.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
