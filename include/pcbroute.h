@@ -89,6 +89,18 @@ typedef struct pcbr_options {
     int32_t ml_rounds;        /* the learned prior (network or file) guides a net's searches in the first negotiation
                                  round only (0, the default) or in every round (1, as src/router.cpp:674,1218 keep
                                  a net's map for all its searches); never while legalizing */
+    /* The GPU gate (src/router.cpp:1523-1524,1690-1701). A search runs on the CPU (pcbgpu_cpu_search's A*) and is
+     * finished on the device once it has made gpu_pops heap pops; 1 (the default): every search goes to the device
+     * at once; 0: never (CPU only, up to max_pops). Both give the same route: the CPU search's result is the device's
+     * canonical one. */
+    int64_t gpu_pops;
+    int64_t max_pops;         /* a CPU search's budget without the device (src/router.h:16: 3,000,000); a connection
+                                 it cannot finish stays open */
+    int64_t probe_pops;       /* before searching for the last open pin of a net, a search from that pin back towards
+                                 the net's copper within this many pops (src/router.cpp:1785-1800; 0: none): a pin
+                                 walled in is left open without a full search */
+    int32_t gpu_wait;         /* wait for a free device slot (1, the default) or, when all are busy, go on on the
+                                 CPU for 200,000 pops and try again (0, --gpu-nowait) */
 } pcbr_options;
 void pcbr_default_options(pcbr_options* o);
 
@@ -99,6 +111,12 @@ typedef struct pcbr_result {
     double route_s, search_s, prior_s, upload_s, backtrace_s;
     double host_s[4];         /* host time: cost rebuilds, stamps, read sets, query builds */
     uint64_t route_hash;      /* FNV-1a over every net's routed cells in order: equal routes, equal hash */
+    /* where the searches ended (the gate): found or exhausted on the CPU, finished on the device, device failures
+     * that the CPU finished, chains the via spacing rejected (searched again on the CPU under the rules), pins the
+     * probe found walled in, connections left open when a CPU budget ran out; heap pops and time on the CPU */
+    int64_t cpu_searches, gpu_searches, gpu_failures, chain_rejects, probe_opens, budget_opens;
+    uint64_t cpu_pops;
+    double cpu_s;
 } pcbr_result;
 
 typedef struct pcbr_router pcbr_router;

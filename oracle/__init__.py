@@ -22,17 +22,18 @@ ROUTE_LIB = os.path.join(HERE, "libroute_oracle.so")
 def build_route_oracle(verbose=False):
     """The router (pcbroute_amd/csrc/router.cpp) over the oracle's search: oracle/libroute_oracle.so."""
     root = os.path.dirname(HERE)
-    obj = os.path.join(HERE, "pcb_oracle_route.o")
+    obj = os.path.join(HERE, f"pcb_oracle_route.{os.getpid()}.o")  # one per process: test workers build it at once
     cmds = [["gcc", "-O2", "-std=c11", "-fPIC", "-c", "-o", obj, SRC],
-            ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", ROUTE_LIB + ".tmp",
+            ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", ROUTE_LIB + f".{os.getpid()}.tmp",
              os.path.join(root, "pcbroute_amd", "csrc", "router.cpp"), os.path.join(root, "pcbroute_amd", "csrc", "dsn.cpp"),
+             os.path.join(root, "pcbroute_amd", "csrc", "astar.cpp"), os.path.join(root, "pcbroute_amd", "csrc", "chain.cpp"),
              os.path.join(HERE, "route_on_oracle.cpp"), obj,
              "-lpthread"]]
     for cmd in cmds:
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-    os.replace(ROUTE_LIB + ".tmp", ROUTE_LIB)
+    os.replace(ROUTE_LIB + f".{os.getpid()}.tmp", ROUTE_LIB)
     os.remove(obj)
     return ROUTE_LIB
 

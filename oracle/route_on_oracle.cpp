@@ -27,6 +27,10 @@ struct pcbgpu_round {
 
 namespace {
 std::string g_err;
+// failures planted by the tests (route_oracle_inject): every k-th solve fails, every k-th backtrace is rejected
+int g_fail_every = 0, g_chain_every = 0;
+long g_solves = 0, g_traces = 0;
+bool planted_failure() { return g_fail_every > 0 && ++g_solves % g_fail_every == 0; }
 
 pcbgpu_query on_grid(const pcbgpu_grid* g, const pcbgpu_query* q) {
     pcbgpu_query x = *q;
@@ -88,7 +92,14 @@ int pcbgpu_grid_upload_rects(pcbgpu_grid* g, int32_t n, const int32_t* rects, co
     return 0;
 }
 
+void route_oracle_inject(int fail_every, int chain_every) {
+    g_fail_every = fail_every;
+    g_chain_every = chain_every;
+    g_solves = g_traces = 0;
+}
+
 int pcbgpu_solve_grid(const pcbgpu_grid* g, const pcbgpu_query* q, pcbgpu_out* out, int) {
+    if (planted_failure()) { *out = pcbgpu_out{}; g_err = "planted failure"; return PCBGPU_ERR_FAIL; }
     const pcbgpu_query x = on_grid(g, q);
     return orc_solve(&x, out, size_t(1) << 31);
 }
@@ -109,7 +120,7 @@ int pcbgpu_round_launch(pcbgpu_round* r) {
     r->rc.assign(r->q.size(), -1);
     for (size_t k = 0; k < r->q.size(); ++k) {
         const pcbgpu_query x = on_grid(r->q[k].first, &r->q[k].second);
-        r->rc[k] = orc_solve(&x, &r->out[k], size_t(1) << 31);
+        r->rc[k] = planted_failure() ? PCBGPU_ERR_FAIL : orc_solve(&x, &r->out[k], size_t(1) << 31);
     }
     return 0;
 }
@@ -128,6 +139,7 @@ void pcbgpu_round_end(pcbgpu_round* r) {
 int64_t pcbgpu_backtrace(const pcbgpu_query* q, const pcbgpu_out* out, int32_t via_gapc, double via_gap2,
                          const uint32_t* old_vias, size_t n_old_vias, uint32_t* chain_states, uint8_t* chain_codes,
                          size_t chain_cap) {
+    if (g_chain_every > 0 && ++g_traces % g_chain_every == 0) return PCBGPU_ERR_CHAIN;
     return orc_backtrace(q, out, via_gapc, via_gap2, old_vias, n_old_vias, chain_states, chain_codes, chain_cap);
 }
 

@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-from .abi import (BLOCK, CF_MAX, DX, DY, ERR_BUSY, ERR_CHAIN, ERR_FAIL, FREE, INF, MAX_LAYERS, NOFIELD, W_MAX,
+from .abi import (BLOCK, CF_MAX, DX, DY, ERR_BUDGET, ERR_BUSY, ERR_CHAIN, ERR_FAIL, FREE, INF, MAX_LAYERS, NOFIELD, W_MAX,
                   Entry, Out, Problem, Query, default_dirw)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -21,6 +21,7 @@ EXPORTS = (
     "pcbgpu_round_result", "pcbgpu_round_end", "pcbgpu_set_tile", "pcbgpu_get_tile", "pcbgpu_set_delta", "pcbgpu_stats_enable",
     "pcbgpu_stats_read", "pcbgpu_last_error", "pcbgpu_autotune", "pcbgpu_tune_get", "pcbgpu_tune_set",
     "pcbgpu_tune_save", "pcbgpu_tune_load", "pcbgpu_grid_upload_rects", "pcbgpu_solve_grid_noalloc",
+    "pcbgpu_cpu_search",
 )
 
 
@@ -73,6 +74,11 @@ def load_library(path=LIB_PATH):
     lib.pcbgpu_solve_grid_noalloc.argtypes = [C.c_void_p, C.POINTER(Query), C.POINTER(C.c_int32), C.c_size_t,
                                               C.POINTER(Out)]
     lib.pcbgpu_solve_grid_noalloc.restype = C.c_int
+    lib.pcbgpu_cpu_search.argtypes = [C.POINTER(Query), C.c_int, C.POINTER(C.c_int32), C.c_size_t, C.c_size_t,
+                                      C.c_int32, C.c_double, C.POINTER(C.c_uint32), C.c_size_t,
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.c_size_t]
+    lib.pcbgpu_cpu_search.restype = C.c_int64
     lib.pcbgpu_set_tile.argtypes = [C.c_int32, C.c_int32]
     lib.pcbgpu_set_tile.restype = C.c_int
     lib.pcbgpu_get_tile.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -105,6 +111,27 @@ def load_library(path=LIB_PATH):
     lib.pcbgpu_tune_load.argtypes = [C.c_char_p]
     lib.pcbgpu_tune_load.restype = C.c_int
     return lib
+
+
+def cpu_search(prob, heuristic=0, boxes=None, max_pops=0, via_gapc=0, via_gap2=0.0, old_vias=(), lib=None):
+    """The CPU search (pcbgpu_cpu_search, host code: no device needed). Returns dict(rc, best, opt, pops, chain):
+    rc > 0 the chain length with chain [(window state, code)] target first, 0 no target reachable, ERR_BUDGET the
+    budget ran out, ERR_CHAIN the via rules rejected the chain. boxes: rows (l, i0, j0, i1, j1) or None."""
+    lib = lib or load_library()
+    WN = prob.window[2] * prob.window[3]
+    cap = WN * prob.nl + 1
+    states = np.zeros(cap, np.uint32)
+    codes = np.zeros(cap, np.uint8)
+    ov = np.asarray(list(old_vias), np.uint32)
+    bx = np.ascontiguousarray(np.asarray(boxes if boxes is not None else np.zeros((0, 5)), np.int32).reshape(-1, 5))
+    best, opt, pops = C.c_int64(), C.c_int32(), C.c_uint64()
+    n = lib.pcbgpu_cpu_search(C.byref(prob.query()), int(heuristic),
+                              bx.ctypes.data_as(C.POINTER(C.c_int32)) if len(bx) else None, len(bx), int(max_pops),
+                              int(via_gapc), float(via_gap2), ov.ctypes.data_as(C.POINTER(C.c_uint32)), len(ov),
+                              C.byref(best), C.byref(opt), C.byref(pops), states.ctypes.data_as(C.POINTER(C.c_uint32)),
+                              codes.ctypes.data_as(C.POINTER(C.c_uint8)), cap)
+    chain = [(int(states[k]), int(codes[k])) for k in range(n)] if n > 0 else []
+    return dict(rc=int(n), best=int(best.value), opt=int(opt.value), pops=int(pops.value), chain=chain)
 
 
 class Result:

@@ -13,7 +13,7 @@ CF_MAX = 0xFFFFF
 W_MAX = 1023
 FREE = 0
 BLOCK = 0xFFFF
-ERR_FAIL, ERR_CHAIN, ERR_BUSY = -1, -2, -3
+ERR_FAIL, ERR_CHAIN, ERR_BUSY, ERR_BUDGET = -1, -2, -3, -4
 NOFIELD = 1
 
 # move codes: 0 +x, 1 -x, 2 +y, 3 -y, 4 (+,+), 5 (+,-), 6 (-,+), 7 (-,-)  (src/router.cpp:1230)

@@ -35,7 +35,8 @@ class Options(C.Structure):
                 ("ml_weight", C.c_double), ("ml_sat", C.c_double), ("ml_corridor", C.c_double),
                 ("ml_margin_mm", C.c_double), ("legalize", C.c_int32), ("verbose", C.c_int32),
                 ("negotiate", C.c_int32), ("rescue", C.c_int32), ("smooth", C.c_int32),
-                ("ml_rounds", C.c_int32)]
+                ("ml_rounds", C.c_int32), ("gpu_pops", C.c_int64), ("max_pops", C.c_int64),
+                ("probe_pops", C.c_int64), ("gpu_wait", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -45,7 +46,10 @@ class Result(C.Structure):
                                          "vias")] + \
                [("tile_visits", C.c_uint64)] + \
                [(k, C.c_double) for k in ("route_s", "search_s", "prior_s", "upload_s", "backtrace_s")] + \
-               [("host_s", C.c_double * 4), ("route_hash", C.c_uint64)]
+               [("host_s", C.c_double * 4), ("route_hash", C.c_uint64)] + \
+               [(k, C.c_int64) for k in ("cpu_searches", "gpu_searches", "gpu_failures", "chain_rejects",
+                                         "probe_opens", "budget_opens")] + \
+               [("cpu_pops", C.c_uint64), ("cpu_s", C.c_double)]
 
 
 SEARCH_CB = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(Query), C.POINTER(Out))
@@ -94,7 +98,7 @@ def library(path=LIB_PATH):
 
 def spec(kind="c4", seed=1, **kw):
     """Board specs of the configurations: 'c1' a 2-layer 40x30 mm board (6 SOIC-16, 20 0805, 60 nets), 'c4' a
-    4-layer 60x50 mm board (2 QFP-64, 8 SOIC-16, 100 0805, 150-300 nets as the pads allow), 'tiny' a 2-layer
+    4-layer 60x50 mm board (2 QFP-64, 8 SOIC-16, 100 or more 0805 so that every one of its 150-300 nets has two pads), 'tiny' a 2-layer
     18x14 mm board for CPU checks. All on a 50 um grid, 0.2 mm track and clearance, 0.6 mm vias."""
     s = BoardSpec()
     s.cell_um, s.n_classes, s.edge_mm, s.seed = 50.0, 1, 0.5, seed
@@ -104,8 +108,10 @@ def spec(kind="c4", seed=1, **kw):
         s.n_soic16, s.n_0805, s.n_qfp64, s.n_nets = 6, 20, 0, 60
     elif kind == "c4":
         s.width_mm, s.height_mm, s.layers = 60.0, 50.0, 4
-        s.n_soic16, s.n_0805, s.n_qfp64 = 8, 100, 2
         s.n_nets = int(np.random.default_rng(seed).integers(150, 301))
+        # two pads for every net at least: more 0805s on boards with more nets
+        s.n_soic16, s.n_qfp64 = 8, 2
+        s.n_0805 = max(100, (2 * s.n_nets + 20 - 8 * 16 - 2 * 64 + 1) // 2)
     elif kind == "tiny":
         s.width_mm, s.height_mm, s.layers = 18.0, 14.0, 2
         s.n_soic16, s.n_0805, s.n_qfp64, s.n_nets = 1, 10, 0, 12

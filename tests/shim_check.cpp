@@ -4,7 +4,7 @@
 #include <cstring>
 #include <random>
 
-#include "gpu.h"
+#include "../pcbroute_amd/csrc/gpu.h"
 
 int main() {
     if (!gpu_available()) { std::printf("no device\n"); return 3; }

@@ -331,12 +331,17 @@ def test_widening_solves_match(backend, oracle):
         g.close()
 
 
-def test_cpp_gpu_h_shim():
-    """The C++ gpu.h shim (GpuQuery / GpuOut / gpu_solve) gives what the C ABI gives."""
+def test_cpp_gpu_h_shim(tmp_path):
+    """The C++ gpu.h shim (GpuQuery / GpuOut / gpu_solve) gives what the C ABI gives: tests/shim_check.cpp, built here
+    with g++ against the library."""
     import os
     import subprocess
-    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "pcbroute_amd", "shim_check")
-    assert os.path.exists(exe), "shim_check is not built (python -m pcbroute_amd.build)"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib = os.path.join(root, "pcbroute_amd")
+    exe = str(tmp_path / "shim_check")
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(root, "tests", "shim_check.cpp"), f"-L{lib}",
+                        "-lpcbgpu", f"-Wl,-rpath,{lib}"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
     _say("cpp shim")
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout + r.stderr

@@ -206,3 +206,51 @@ def test_prior_file_maps_take_effect(lib, tmp_path):
     r.read_prior(path)
     res = r.route()
     assert res["search_fallbacks"] > 0 and res["violations"] == 0
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_gate_does_not_change_routes(lib, seed):
+    """The GPU gate (src/router.cpp:1523-1524,1690-1701): every search on the back end at once (gpu_pops 1), none
+    (0: the CPU search alone) or handed over after 300 pops give the same routes, since the CPU search returns the
+    canonical result; with waves the CPU part runs on host threads and the routes still equal the fixture's."""
+    g = [x for x in json.load(open(GOLDEN)) if x["kind"] == "tiny" and x["seed"] == seed][0]
+    b = R.Board(R.spec("tiny", seed=seed), lib_path=lib)
+    L = R.library(lib)
+    for gp, wave in [(0, 0), (300, 0), (0, 5), (300, -8)]:
+        res = R.Router(b, R.default_options(L, gpu_pops=gp, wave=wave), lib_path=lib).route()
+        assert res["route_hash"] == g["route_hash"], (gp, wave)
+        assert res["cpu_searches"] + res["gpu_searches"] == res["searches"]
+        if gp == 0:
+            assert res["gpu_searches"] == 0 and res["cpu_pops"] > 0
+    allcpu = R.Router(b, R.default_options(L, gpu_pops=0), lib_path=lib).route()
+    assert allcpu["cpu_searches"] == allcpu["searches"] == g["searches"]
+
+
+def test_back_end_failures_degrade_to_the_cpu(lib):
+    """A failing back-end solve (-1) and a rejected chain (-2) no longer abort the route (src/router.cpp:1597-1598,
+    1654-1677): the CPU search takes the connection over, and the board still routes clean."""
+    import ctypes as C
+    L = R.library(lib)
+    L.route_oracle_inject.argtypes = [C.c_int, C.c_int]
+    b = R.Board(R.spec("tiny", seed=2), lib_path=lib)
+    try:
+        L.route_oracle_inject(3, 0)
+        r = R.Router(b, R.default_options(L), lib_path=lib)
+        res = r.route()
+        assert res["gpu_failures"] > 0 and res["opens"] == 0 and r.eval() == (0, 0)
+        L.route_oracle_inject(0, 4)
+        r = R.Router(b, R.default_options(L), lib_path=lib)
+        res = r.route()
+        assert res["chain_rejects"] > 0 and res["opens"] == 0 and r.eval() == (0, 0)
+    finally:
+        L.route_oracle_inject(0, 0)
+
+
+def test_probe_leaves_walled_in_pins_as_the_full_search_does(lib):
+    """The probe from a net's last open pin towards its copper (src/router.cpp:1785-1800) only ever answers
+    'unreachable' where the full search finds nothing: the routes equal those without it."""
+    b = R.Board(R.spec("tiny", seed=3), lib_path=lib)
+    L = R.library(lib)
+    plain = R.Router(b, R.default_options(L, gpu_pops=0), lib_path=lib).route()
+    probed = R.Router(b, R.default_options(L, gpu_pops=0, probe_pops=5000), lib_path=lib).route()
+    assert probed["route_hash"] == plain["route_hash"] and probed["opens"] == plain["opens"]
