@@ -122,14 +122,18 @@ int64_t pcbgpu_backtrace(const pcbgpu_query* q, const pcbgpu_out* out, int32_t v
  * heuristic 0: the cheapest walk over an empty grid to the nearest target box, plus a via where the box is on another
  * layer (boxes: n_boxes rows of 5 int32 {l, i0, j0, i1, j1}, window cells, inclusive, l < 0 every layer, each target
  * inside one; NULL: one box per layer around its targets); 1: a distance map over blocks of 8 x 8 cells
- * (src/router.cpp:1401-1492). At most max_pops heap pops (0: no limit). Returns the chain length (> 0) with best, opt
+ * (src/router.cpp:1401-1492). cf_min: a least cost factor of every state and via of the window that the caller
+ * guarantees (1 always holds; a router whose maps never go below one uncongested cell passes that), which scales the
+ * estimates; a search that meets a smaller factor fails (PCBGPU_ERR_FAIL). At most max_pops heap pops (0: no
+ * limit). Returns the chain length (> 0) with best, opt
  * and the chain written as pcbgpu_backtrace writes it; 0 when no target can be reached; PCBGPU_ERR_BUDGET when the
  * pops ran out; PCBGPU_ERR_CHAIN when the via rules reject the chain; PCBGPU_ERR_FAIL on a bad query. pops gets the
  * heap pops made. Host code only: it runs without a device. */
 #define PCBGPU_ERR_BUDGET (-4)
-int64_t pcbgpu_cpu_search(const pcbgpu_query* q, int heuristic, const int32_t* boxes, size_t n_boxes, size_t max_pops,
-                          int32_t via_gapc, double via_gap2, const uint32_t* old_vias, size_t n_old_vias, int64_t* best,
-                          int32_t* opt, uint64_t* pops, uint32_t* chain_states, uint8_t* chain_codes, size_t chain_cap);
+int64_t pcbgpu_cpu_search(const pcbgpu_query* q, int heuristic, const int32_t* boxes, size_t n_boxes, int32_t cf_min,
+                          size_t max_pops, int32_t via_gapc, double via_gap2, const uint32_t* old_vias,
+                          size_t n_old_vias, int64_t* best, int32_t* opt, uint64_t* pops, uint32_t* chain_states,
+                          uint8_t* chain_codes, size_t chain_cap);
 
 /* ---- device-resident grids (no counterpart in the reference: it copies the host maps for every solve,
  * src/gpu.cu:432-439). The router's per-class cost maps live in HBM and are kept in step by span updates; a solve

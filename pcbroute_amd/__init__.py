@@ -74,7 +74,7 @@ def load_library(path=LIB_PATH):
     lib.pcbgpu_solve_grid_noalloc.argtypes = [C.c_void_p, C.POINTER(Query), C.POINTER(C.c_int32), C.c_size_t,
                                               C.POINTER(Out)]
     lib.pcbgpu_solve_grid_noalloc.restype = C.c_int
-    lib.pcbgpu_cpu_search.argtypes = [C.POINTER(Query), C.c_int, C.POINTER(C.c_int32), C.c_size_t, C.c_size_t,
+    lib.pcbgpu_cpu_search.argtypes = [C.POINTER(Query), C.c_int, C.POINTER(C.c_int32), C.c_size_t, C.c_int32, C.c_size_t,
                                       C.c_int32, C.c_double, C.POINTER(C.c_uint32), C.c_size_t,
                                       C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.c_size_t]
@@ -113,7 +113,7 @@ def load_library(path=LIB_PATH):
     return lib
 
 
-def cpu_search(prob, heuristic=0, boxes=None, max_pops=0, via_gapc=0, via_gap2=0.0, old_vias=(), lib=None):
+def cpu_search(prob, heuristic=0, boxes=None, max_pops=0, via_gapc=0, via_gap2=0.0, old_vias=(), cf_min=1, lib=None):
     """The CPU search (pcbgpu_cpu_search, host code: no device needed). Returns dict(rc, best, opt, pops, chain):
     rc > 0 the chain length with chain [(window state, code)] target first, 0 no target reachable, ERR_BUDGET the
     budget ran out, ERR_CHAIN the via rules rejected the chain. boxes: rows (l, i0, j0, i1, j1) or None."""
@@ -126,7 +126,8 @@ def cpu_search(prob, heuristic=0, boxes=None, max_pops=0, via_gapc=0, via_gap2=0
     bx = np.ascontiguousarray(np.asarray(boxes if boxes is not None else np.zeros((0, 5)), np.int32).reshape(-1, 5))
     best, opt, pops = C.c_int64(), C.c_int32(), C.c_uint64()
     n = lib.pcbgpu_cpu_search(C.byref(prob.query()), int(heuristic),
-                              bx.ctypes.data_as(C.POINTER(C.c_int32)) if len(bx) else None, len(bx), int(max_pops),
+                              bx.ctypes.data_as(C.POINTER(C.c_int32)) if len(bx) else None, len(bx), int(cf_min),
+                              int(max_pops),
                               int(via_gapc), float(via_gap2), ov.ctypes.data_as(C.POINTER(C.c_uint32)), len(ov),
                               C.byref(best), C.byref(opt), C.byref(pops), states.ctypes.data_as(C.POINTER(C.c_uint32)),
                               codes.ctypes.data_as(C.POINTER(C.c_uint8)), cap)

@@ -94,11 +94,14 @@ void Astar::build_coarse() {
     }
 }
 
-bool Astar::begin(const pcbgpu_query* q, Heuristic hk, const TargetBox* boxes, size_t nbox, bool reverse) {
+bool Astar::begin(const pcbgpu_query* q, Heuristic hk, const TargetBox* boxes, size_t nbox, bool reverse,
+                  int32_t cf_min) {
     best = -1;
     opt = INF;
     pops = 0;
     found_ = false;
+    broken_ = false;
+    cfmin_ = std::clamp<int32_t>(cf_min, 1, PCBGPU_CF_MAX);
     rules_ = false;
     heap_.clear();
     if (!qc_.init(q)) return false;
@@ -119,8 +122,13 @@ bool Astar::begin(const pcbgpu_query* q, Heuristic hk, const TargetBox* boxes, s
         wd_ = std::min({wd_, q->dirw[l][4], q->dirw[l][5], q->dirw[l][6], q->dirw[l][7]});
     }
     if (!any) wx_ = wy_ = wd_ = 0;
+    // every step costs at least its weight times the least factor (weights <= 1023, factors < 2^20: no overflow in
+    // the 64-bit sums of h; the products are capped below INF)
+    wx_ = int32_t(std::min<int64_t>(int64_t(wx_) * cfmin_, INF - 1));
+    wy_ = int32_t(std::min<int64_t>(int64_t(wy_) * cfmin_, INF - 1));
+    wd_ = int32_t(std::min<int64_t>(int64_t(wd_) * cfmin_, INF - 1));
     wmax_ = std::max({wx_, wy_, int32_t(std::min<int64_t>(wd_, int64_t(wx_) + wy_))});
-    vmin_ = q->via_cost >= 0 ? q->via_cost : INF;  // a via's factor is at least 1
+    vmin_ = q->via_cost >= 0 ? int32_t(std::min<int64_t>(int64_t(q->via_cost) * cfmin_, INF - 1)) : INF;
     // per-state arrays, grown as needed; epochs make a fresh search without clearing them
     const size_t NS = size_t(qc_.NS);
     if (g_.size() < NS) {
@@ -246,7 +254,7 @@ Astar::Status Astar::run(size_t budget) {
     size_t used = 0;
     while (!heap_.empty()) {
         if (found_ && heap_.front().f > opt) break;  // every state that can tie the optimum is done
-        if (budget && used >= budget) return BUDGET;
+        if (budget && used >= budget) return broken_ ? BAD : BUDGET;
         std::pop_heap(heap_.begin(), heap_.end(), ItemLess());
         const Item it = heap_.back();
         heap_.pop_back();
@@ -276,12 +284,14 @@ Astar::Status Astar::run(size_t budget) {
                     if (ni < 0 || nj < 0 || ni >= ww || nj >= wh) continue;
                     const uint32_t w = qc_.word(l, ni, nj);
                     if (w & BLK_BIT) continue;
+                    broken_ |= int32_t(w & CF_MASK) < cfmin_;
                     push(uint32_t(int64_t(l) * WN + int64_t(nj) * ww + ni), g + q->dirw[l][dm] * int32_t(w & CF_MASK), l,
                          ni, nj, uint8_t(dm), rstep);
                 }
             const bool via_ok = !rules_ || ((rq == 255 || rq >= gapc_) && !via_blocked(i, j));
             if (q->via_cost >= 0 && via_ok) {
                 const int32_t vc = qc_.via(i, j);
+                broken_ |= vc < vmin_;
                 if (vc < INF)
                     for (int l2 = 0; l2 < nl; ++l2) {
                         if (l2 == l || !(qc_.word(l2, i, j) & VOK_BIT)) continue;
@@ -291,6 +301,7 @@ Astar::Status Astar::run(size_t budget) {
         } else {
             // reversed: a forward move from a neighbour into this state exists only if this state may be entered
             const uint32_t w = qc_.word(l, i, j);
+            broken_ |= !(w & BLK_BIT) && int32_t(w & CF_MASK) < cfmin_;
             if (q->lay[l] && !(w & BLK_BIT))
                 for (int dm = 0; dm < 8; ++dm) {
                     const int pi = i - DXS[dm], pj = j - DYS[dm];
@@ -300,12 +311,14 @@ Astar::Status Astar::run(size_t budget) {
                 }
             if (q->via_cost >= 0 && (w & VOK_BIT)) {
                 const int32_t vc = qc_.via(i, j);
+                broken_ |= vc < vmin_;
                 if (vc < INF)
                     for (int l2 = 0; l2 < nl; ++l2)
                         if (l2 != l) push(uint32_t(int64_t(l2) * WN + c), g + vc, l2, i, j);
             }
         }
     }
+    if (broken_) return BAD;  // the estimate may have overshot: the result is not known to be the least
     return found_ ? FOUND : EXHAUSTED;
 }
 
@@ -314,7 +327,8 @@ Astar::Status Astar::run(size_t budget) {
 // ---------------------------------------------------------------- C ABI
 
 extern "C" int64_t pcbgpu_cpu_search(const pcbgpu_query* q, int heuristic, const int32_t* boxes, size_t n_boxes,
-                                     size_t max_pops, int32_t via_gapc, double via_gap2, const uint32_t* old_vias,
+                                     int32_t cf_min, size_t max_pops, int32_t via_gapc, double via_gap2,
+                                     const uint32_t* old_vias,
                                      size_t n_old_vias, int64_t* best, int32_t* opt, uint64_t* pops,
                                      uint32_t* chain_states, uint8_t* chain_codes, size_t chain_cap) {
     thread_local pcb::Astar A;
@@ -327,7 +341,8 @@ extern "C" int64_t pcbgpu_cpu_search(const pcbgpu_query* q, int heuristic, const
         const int32_t* b = boxes + 5 * k;
         bx.push_back({b[0], b[1], b[2], b[3], b[4]});
     }
-    if (!A.begin(q, heuristic ? pcb::Astar::COARSE : pcb::Astar::BOXES, bx.data(), bx.size())) return PCBGPU_ERR_FAIL;
+    if (!A.begin(q, heuristic ? pcb::Astar::COARSE : pcb::Astar::BOXES, bx.data(), bx.size(), false, cf_min))
+        return PCBGPU_ERR_FAIL;
     const pcb::Astar::Status st = A.run(max_pops);
     if (pops) *pops = A.pops;
     if (st == pcb::Astar::BUDGET) return PCBGPU_ERR_BUDGET;

@@ -36,7 +36,11 @@ public:
     // moves of the query's costs (a move may leave a state only where a forward move may enter it), so it exhausts
     // exactly when no forward search from the targets could reach the starts. The router's probe uses it for that
     // answer only: its costs are those of the forward moves it reverses, not a canonical result.
-    bool begin(const pcbgpu_query* q, Heuristic hk, const TargetBox* boxes, size_t nbox, bool reverse = false);
+    // cf_min: a lower bound of every cost factor of the window (track and via) that the caller guarantees; the
+    // estimates scale with it (the router's maps never go below one uncongested cell's factor). A search that meets
+    // a smaller factor ends with BAD rather than a result that might not be the least.
+    bool begin(const pcbgpu_query* q, Heuristic hk, const TargetBox* boxes, size_t nbox, bool reverse = false,
+               int32_t cf_min = 1);
     // The reference's via spacing inside the search (src/router.cpp:1747-1756), set after begin and before run: a
     // via may leave a state only gapc or more track steps after the last via of the path that reached it, and not
     // at a cell (full grid) closer than sqrt(gap2) to one of old_vias other than itself. The search then keeps one
@@ -76,7 +80,9 @@ private:
     Heuristic hk_ = BOXES;
     bool reverse_ = false, found_ = false;
     std::vector<TargetBox> boxes_;
-    int32_t wx_ = 1, wy_ = 1, wd_ = 1, wmax_ = 1, vmin_ = INF;  // least move weights; least via cost
+    int32_t wx_ = 1, wy_ = 1, wd_ = 1, wmax_ = 1, vmin_ = INF;  // least move costs (weight x least factor); least via cost
+    int32_t cfmin_ = 1;
+    bool broken_ = false;  // a factor below cfmin_ was met
     // coarse map
     static constexpr int CS = 8;
     int cw_ = 0, ch_ = 0;

@@ -112,8 +112,26 @@ def test_bad_queries_are_refused():
     best, opt, pops = C.c_int64(), C.c_int32(), C.c_uint64()
     states = np.zeros(8, np.uint32)
     codes = np.zeros(8, np.uint8)
-    rc = lib.pcbgpu_cpu_search(C.byref(q), 0, None, 0, 0, 0, 0.0, None, 0, C.byref(best), C.byref(opt), C.byref(pops),
+    rc = lib.pcbgpu_cpu_search(C.byref(q), 0, None, 0, 1, 0, 0, 0.0, None, 0, C.byref(best), C.byref(opt), C.byref(pops),
                                states.ctypes.data_as(C.POINTER(C.c_uint32)), codes.ctypes.data_as(C.POINTER(C.c_uint8)),
                                8)
     assert rc == ERR_FAIL
     assert cpu_search(p, heuristic=7)["rc"] == ERR_FAIL
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_least_factor_scales_the_estimate(oracle, seed):
+    """Cost maps whose factors are all at least 8 (as the router's are): with cf_min 8 the same answer as with 1 in
+    fewer pops; a cf_min above a factor the search meets fails instead of answering."""
+    p = random_case(2500 + seed, nx=140, ny=100, nl=2 + seed % 2, features=False)
+    cst = np.where(p.cst < INF, p.cst * 8, p.cst).astype(np.int32)
+    vcst = np.where(p.vcst < INF, p.vcst * 8, p.vcst).astype(np.int32)
+    q = Problem(cst, p.st, vcst, p.vst, starts=p.starts, targets=p.targets, window=p.window, via_cost=p.via_cost,
+                dirw=p.dirw, lay=p.lay[:p.nl])
+    ref, ans = _oracle_answer(oracle, q)
+    one, eight = cpu_search(q, cf_min=1), cpu_search(q, cf_min=8)
+    assert (one["rc"], one["best"], one["opt"], one["chain"]) == (eight["rc"], eight["best"], eight["opt"], eight["chain"])
+    if ans is not None:
+        assert eight["opt"] == ans[0] and eight["chain"] == ans[1]
+        assert eight["pops"] < one["pops"]
+        assert cpu_search(q, cf_min=9)["rc"] == ERR_FAIL
