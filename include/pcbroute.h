@@ -117,6 +117,9 @@ typedef struct pcbr_result {
     int64_t cpu_searches, gpu_searches, gpu_failures, chain_rejects, probe_opens, budget_opens;
     uint64_t cpu_pops;
     double cpu_s;
+    int32_t first_conflict_nets, first_conflict_cells;  /* left in conflict by the first negotiation round */
+    uint64_t solved_states;   /* states the searches covered: each device search's solved rectangle, each CPU
+                                 search's window (the work a search costs, whichever side ran it) */
 } pcbr_result;
 
 typedef struct pcbr_router pcbr_router;

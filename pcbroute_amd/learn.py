@@ -110,9 +110,22 @@ CONFIGS = {  # name: router options, and "prior": route with the network's prior
     "parallel_learned_slow": dict(wave=64, negotiate=1, patience=12, prior=True),
     "parallel_learned_hist": dict(wave=64, negotiate=1, patience=12, hist_inc=1.0, pres_growth=2.0, prior=True),
     "parallel16_learned_hist": dict(wave=16, negotiate=1, patience=12, hist_inc=1.0, pres_growth=2.0, prior=True),
+    # looser corridors, the bias alone (the window then the pins' box plus the margin), wider margins
+    "learned_c02": dict(wave=0, ml_corridor=0.02, prior=True),
+    "learned_c01": dict(wave=0, ml_corridor=0.01, prior=True),
+    "learned_bias": dict(wave=0, ml_corridor=0.0, prior=True),
+    "learned_m3": dict(wave=0, ml_margin_mm=3.0, prior=True),
+    "learned_c02_all": dict(wave=0, ml_corridor=0.02, ml_rounds=1, prior=True),
+    "grouped_learned_c02": dict(wave=-64, ml_corridor=0.02, prior=True),
+    "plain_cpu": dict(wave=0, gpu_pops=0),                 # the CPU search alone
+    "learned_cpu": dict(wave=0, gpu_pops=0, prior=True),
+    "plain_p2": dict(wave=0, patience=2),
+    "learned_p2": dict(wave=0, patience=2, prior=True),
+    "learned_p3": dict(wave=0, patience=3, prior=True),
 }
 FIELDS = ("opens", "violations", "route_s", "search_s", "prior_s", "upload_s", "backtrace_s", "host_s", "searches",
-          "rounds", "revalidated", "iterations", "search_fallbacks", "length_cells", "vias")
+          "rounds", "revalidated", "iterations", "search_fallbacks", "length_cells", "vias", "first_conflict_nets",
+          "first_conflict_cells", "solved_states", "cpu_pops")
 
 
 REF = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "pcbroute")

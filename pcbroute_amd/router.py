@@ -49,7 +49,8 @@ class Result(C.Structure):
                [("host_s", C.c_double * 4), ("route_hash", C.c_uint64)] + \
                [(k, C.c_int64) for k in ("cpu_searches", "gpu_searches", "gpu_failures", "chain_rejects",
                                          "probe_opens", "budget_opens")] + \
-               [("cpu_pops", C.c_uint64), ("cpu_s", C.c_double)]
+               [("cpu_pops", C.c_uint64), ("cpu_s", C.c_double), ("first_conflict_nets", C.c_int32),
+                ("first_conflict_cells", C.c_int32), ("solved_states", C.c_uint64)]
 
 
 SEARCH_CB = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(Query), C.POINTER(Out))
