@@ -6,8 +6,10 @@ The searches of a step run as one round (one sequence of launches shared by all 
 the same searches one after another (one solve at a time) are timed beside it. Prints one JSON line.
 
 value: grid states solved per second (64 x 12.8M states per step / wall time of the round mode).
-roofline: the sweep kernel k_pass, algorithmic bytes = 12 B per state carried through a tile visit (int32 cost word
-read, int32 field read and write) / summed kernel time from HIP events on the launch stream.
+roofline: the sweep kernel k_pass, algorithmic bytes = 12 B per state per tile visit (int32 cost word read, int32
+field read and write, once per visit however many local sweeps it runs) / summed kernel time from HIP events on the
+launch stream. Beside it: the HBM bytes rocprofv3's FETCH_SIZE and WRITE_SIZE counters measured per launch of the
+same configuration (profiles/), their rate, and the figure that credits every local sweep with 12 B.
 Tile rows, local sweeps and the delta band: the library's defaults for this workload (16-row tiles, up to 4 local
 sweeps, automatic band), so the committed rocprof traffic profile matches the run; --tune auto runs the autotuner on
 a sample drawn with another seed first. The batched round workload tunes by default.
@@ -341,7 +343,7 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--local-sweeps", type=int, default=0)
     ap.add_argument("--delta", type=int, default=DEFAULT_DELTA, help="delta-stepping band over tiles (0: automatic, -1: off)")
-    ap.add_argument("--cpu-sample", type=int, default=32)
+    ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=("both", "single", "round"), default="both")
     ap.add_argument("--workload", choices=("kernel", "round", "prior", "c1", "c5"), default="kernel",
@@ -411,10 +413,12 @@ def main():
     n_search = a.steps * len(probs)
     cells = nx * ny * nl
     kern_s = st["kernel_ms"] / 1e3
-    # the algorithmic unit: a state-sweep, 12 bytes (cost read, field read, field write); each local sweep of a tile
-    # visit carries every state of the tile through one
-    achieved = st["state_sweeps"] * ALGO_BYTES / kern_s / 1e9 if kern_s > 0 else 0.0
-    per_visit = st["states"] * ALGO_BYTES / kern_s / 1e9 if kern_s > 0 else 0.0
+    # the algorithmic bytes: 12 per state per tile visit (cost read, field read, field write); a visit's local sweeps
+    # work on the tile in registers and LDS, so crediting each of them with 12 B (the state-sweep figure) is reported
+    # beside it only
+    achieved = st["states"] * ALGO_BYTES / kern_s / 1e9 if kern_s > 0 else 0.0
+    per_sweep = st["state_sweeps"] * ALGO_BYTES / kern_s / 1e9 if kern_s > 0 else 0.0
+    avg_us = st["kernel_ms"] * 1e3 / max(1, st["launches"])
     line = {
         "metric": "grid-search Gcells/s (% HBM roof)",
         "value": round(n_search * cells / wall / 1e9, 3),
@@ -439,15 +443,15 @@ def main():
         "passes_per_search": round(sum(sweeps) / len(sweeps), 2),
         "tile_state_visits_per_search": round(st["states"] / n_search / cells, 2),
         "state_sweeps_per_search": round(st["state_sweeps"] / n_search / cells, 2),
-        "gb_s_counting_each_visit_once": round(per_visit, 1),
+        "gb_s_counting_each_local_sweep": round(per_sweep, 1),
         "processed_gstates_per_s": round(st["states"] / wall / 1e9, 3),
         "rect_state_passes_gcells_s": round(st["rect_state_passes"] / wall / 1e9, 3),
         "pct_hbm_roof": round(100 * achieved / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_pass", "launches": st["launches"],
-                     "avg_launch_us": round(st["kernel_ms"] * 1e3 / max(1, st["launches"]), 3),
-                     "algo_bytes_per_launch": round(st["state_sweeps"] * ALGO_BYTES / max(1, st["launches"]))},
+                     "avg_launch_us": round(avg_us, 3),
+                     "algo_bytes_per_launch": round(st["states"] * ALGO_BYTES / max(1, st["launches"]))},
     }
     # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this configuration
     tpath = os.path.join(ROOT, "profiles", "k_pass_kernel_board_2000x1600x4_round.traffic.json")
@@ -455,8 +459,13 @@ def main():
         t = json.load(open(tpath))
         want = {"tile_rows": rows, "local_sweeps": lsw, "delta": delta, "mode": head, "pairs": len(probs)}
         if t.get("config") == want:
-            line["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
-            line["roofline"]["traffic_source"] = t["source"]
+            rl = line["roofline"]
+            rl["traffic"] = t["traffic_bytes_per_launch"]
+            rl["traffic_source"] = t["source"]
+            # the counters' bytes at this run's launch time, and how far they exceed the algorithmic bytes
+            rl["measured_gb_s"] = round(t["traffic_bytes_per_launch"] / (avg_us * 1e3), 1) if avg_us > 0 else None
+            rl["measured_frac"] = round(rl["measured_gb_s"] / HBM_PEAK_GBS, 4) if avg_us > 0 else None
+            rl["traffic_over_algorithmic"] = round(t["traffic_bytes_per_launch"] / max(1, rl["algo_bytes_per_launch"]), 3)
     # the same round with every search's canonical field copied back into pinned host memory (what a caller that
     # backtraces on the host pays): reported beside the value, never as it
     if a.workload == "kernel":
@@ -471,8 +480,12 @@ def main():
                                  "gcells_s": round(n_search * cells / w1 / 1e9, 3),
                                  "passes_per_search": round(sum(sw1) / len(sw1), 2)}
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # every host core this process may use, up to the 16 a one-GPU share of the box holds
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = max(1, min(16, avail))
         cb = cpu_baseline(board, a.pairs_seed, min(a.cpu_sample, a.pairs), threads, a.pairs)
+        cb["nproc"] = os.cpu_count()
+        cb["cores_available"] = avail
         line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 
