@@ -325,6 +325,52 @@ def prior_workload(b, a):
     print(json.dumps(line), flush=True)
 
 
+def c5board_workload(b, a):
+    """C5: the 8-layer 160x120 mm board (two reserved inner planes, four BGA-256 whose nets escape in the 0.1 mm class,
+    10 QFP-64, 30 SOIC-16, 400 0805, 1200 nets, three net classes) routed by the router on the GPU under a 240 s time
+    limit, plain (one net at a time) and batched (each box-disjoint group of the order routed as one GPU round per
+    step, commits validated in order); opens and violations by the router's own check and, where the reference's
+    evaluator is built, by it. There is no learned prior for 8 layers: the network is trained for the 4-layer C4
+    boards."""
+    import subprocess
+    import tempfile
+    from pcbroute_amd import router as R
+    board = R.Board(R.spec("c5", seed=a.board_seed))
+    i = board.info
+    log(f"c5 board {i.nx}x{i.ny}x{i.nl}: {i.n_nets} nets, {i.n_connections} connections, {i.n_parts} parts")
+    ref = os.path.join(ROOT, "oracle", "_ref", "pcbroute")
+    rows = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        dsn = os.path.join(tmp, "c5.dsn")
+        board.write_dsn(dsn)
+        for name, opts in (("plain", dict(wave=0)), ("batched", dict(wave=-64))):
+            r = R.Router(board, R.default_options(time_limit_s=240.0, **opts))
+            t0 = time.perf_counter()
+            res = r.route()
+            wall = time.perf_counter() - t0
+            opens, viol = r.eval()
+            row = {"route_s": round(wall, 2), "opens": res["opens"], "violations": viol, "eval_opens": opens,
+                   "iterations": res["iterations"], "searches": res["searches"], "gpu_rounds": res["rounds"],
+                   "limited": res["limited"], "rescue_kept": res["rescue_kept"],
+                   "opens_before_rescue": res["opens_before_rescue"], "vias": res["vias"],
+                   "search_s": round(res["search_s"], 2)}
+            if os.path.exists(ref):
+                ses = os.path.join(tmp, name + ".ses")
+                r.write_ses(ses)
+                row["reference_judge"] = _ref_eval(ref, dsn, ses)
+            rows[name] = row
+            log(f"c5 {name}: {row}")
+            r.close()
+    print(json.dumps({
+        "metric": "board route s, batched vs plain, same opens", "value": rows["batched"]["route_s"], "unit": "s",
+        "n_gpus": 1, "higher_is_better": False, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "C5 board: 3200x2400x8 (two reserved planes), 4 BGA-256, 10 QFP-64, 30 SOIC-16, 400 "
+                               "0805, three net classes, 240 s limit", "board_seed": a.board_seed,
+                   "nets": i.n_nets, "connections": i.n_connections},
+        "batched_over_plain_time": round(rows["batched"]["route_s"] / rows["plain"]["route_s"], 3),
+        "plain": rows["plain"], "batched": rows["batched"]}), flush=True)
+
+
 def synth_problem_of(grid):
     """A Problem that only carries a class's maps (to create its device-resident grid)."""
     from pcbroute_amd import Problem
@@ -346,13 +392,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=("both", "single", "round"), default="both")
-    ap.add_argument("--workload", choices=("kernel", "round", "prior", "c1", "c5"), default="kernel",
+    ap.add_argument("--workload", choices=("kernel", "round", "prior", "c1", "c5", "c5board"), default="kernel",
                     help="kernel: the 64 whole-grid searches (the metric); c5: the same on a 3200x2400x8 grid (the "
                          "C5 board's 61.4M states), 16 searches; round: 256 net searches of a speculative "
                          "round on a 2400x1800x6 board with two net classes, as one round and one at a time; "
                          "prior: the corridor CNN's training and inference on full 4-layer boards; c1: the router on "
                          "the C1 board, its session judged by the reference's evaluator, the reference's CPU router "
-                         "timed beside it")
+                         "timed beside it; c5board: the C5 board routed plain and batched under its time limit")
     ap.add_argument("--tune", choices=("auto", "off"), default=None,
                     help="auto: the autotuner picks tile rows, local sweeps and delta on a sample drawn with another "
                          "seed (the default for --workload round); off: library defaults or the explicit flags (the "
@@ -392,6 +438,8 @@ def main():
         return prior_workload(b, a)
     if a.workload == "c1":
         return c1_workload(b, a)
+    if a.workload == "c5board":
+        return c5board_workload(b, a)
     log("building the board")
     dims = dict(nx=3200, ny=2400, nl=8) if a.workload == "c5" else {}
     if a.workload == "c5" and a.pairs == 64:

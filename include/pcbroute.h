@@ -32,6 +32,10 @@ typedef struct pcbr_board_spec {
     double track_mm[PCBR_MAX_CLASSES], clear_mm[PCBR_MAX_CLASSES], via_mm[PCBR_MAX_CLASSES];
     double edge_mm;                       /* copper to board edge */
     uint64_t seed;
+    int32_t n_bga256;                     /* BGA-256 parts: 16 x 16 pads of 0.4 mm at 0.8 mm pitch, placed first; a
+                                             net with a BGA pin takes class 0 (the finest), others round robin */
+    int32_t n_planes;                     /* inner layers reserved as planes (the middle ones): no class's tracks
+                                             use them, vias pass them */
 } pcbr_board_spec;
 
 typedef struct pcbr_board pcbr_board;
@@ -82,8 +86,10 @@ typedef struct pcbr_options {
                                  next round (parallel PathFinder); 2: also in the first round; 3: as 1 until two
                                  rounds in a row bring no 10 % fewer conflicts, then in order as 0 (with a fresh
                                  patience). Legalizing is always one net at a time. */
-    int32_t rescue;           /* while legalizing, a connection the net's window cannot hold is searched again over
-                                 the whole board (1, the default) or left open (0) */
+    int32_t rescue;           /* rounds of the rescue pass after legalizing (src/router.cpp:2401-2454; 1, the default;
+                                 0: none): a net still open is routed again pushing through others' copper, the
+                                 nets it then crosses route around it, and the board is kept only if fewer
+                                 connections are open in all, else put back */
     int32_t smooth;           /* pcbr_write_ses smooths the tracks (1, the default, as src/main.cpp:287-290) or writes
                                  the grid paths' straight runs (0) */
     int32_t ml_rounds;        /* the learned prior (network or file) guides a net's searches in the first negotiation
@@ -101,6 +107,10 @@ typedef struct pcbr_options {
                                  walled in is left open without a full search */
     int32_t gpu_wait;         /* wait for a free device slot (1, the default) or, when all are busy, go on on the
                                  CPU for 200,000 pops and try again (0, --gpu-nowait) */
+    double time_limit_s;      /* a route's time limit (src/router.cpp:2247-2256; 0: none): negotiation stops when the
+                                 next round would pass half of it, legalizing follows, the rescue pass gets what is
+                                 left */
+    double rescue_s;          /* the rescue pass's own cap (0: half the time the route took so far, at least 10 s) */
 } pcbr_options;
 void pcbr_default_options(pcbr_options* o);
 
@@ -118,6 +128,8 @@ typedef struct pcbr_result {
     uint64_t cpu_pops;
     double cpu_s;
     int32_t first_conflict_nets, first_conflict_cells;  /* left in conflict by the first negotiation round */
+    int32_t rescue_tried, rescue_kept, opens_before_rescue;
+    int32_t limited;          /* 1: the time limit cut negotiation short, 2: also the rescue pass */
     uint64_t solved_states;   /* states the searches covered: each device search's solved rectangle, each CPU
                                  search's window (the work a search costs, whichever side ran it) */
 } pcbr_result;

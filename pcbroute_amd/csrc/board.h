@@ -9,7 +9,8 @@
 namespace pcbr {
 
 struct Cls {
-    double w, c, v;  // track width, clearance, via diameter (cells)
+    double w, c, v;                // track width, clearance, via diameter (cells)
+    uint32_t layers = 0xFFFFFFFFu;  // the layers its tracks may use (a reserved plane layer is left out); vias pass all
 };
 struct Pad {
     int x0, y0, x1, y1;  // cells a path may end in (centres inside the copper), half open

@@ -20,7 +20,7 @@ class BoardSpec(C.Structure):
                 ("n_soic16", C.c_int32), ("n_0805", C.c_int32), ("n_qfp64", C.c_int32), ("n_nets", C.c_int32),
                 ("n_classes", C.c_int32), ("track_mm", C.c_double * MAX_CLASSES),
                 ("clear_mm", C.c_double * MAX_CLASSES), ("via_mm", C.c_double * MAX_CLASSES),
-                ("edge_mm", C.c_double), ("seed", C.c_uint64)]
+                ("edge_mm", C.c_double), ("seed", C.c_uint64), ("n_bga256", C.c_int32), ("n_planes", C.c_int32)]
 
 
 class BoardInfo(C.Structure):
@@ -36,7 +36,8 @@ class Options(C.Structure):
                 ("ml_margin_mm", C.c_double), ("legalize", C.c_int32), ("verbose", C.c_int32),
                 ("negotiate", C.c_int32), ("rescue", C.c_int32), ("smooth", C.c_int32),
                 ("ml_rounds", C.c_int32), ("gpu_pops", C.c_int64), ("max_pops", C.c_int64),
-                ("probe_pops", C.c_int64), ("gpu_wait", C.c_int32)]
+                ("probe_pops", C.c_int64), ("gpu_wait", C.c_int32), ("time_limit_s", C.c_double),
+                ("rescue_s", C.c_double)]
 
 
 class Result(C.Structure):
@@ -50,7 +51,8 @@ class Result(C.Structure):
                [(k, C.c_int64) for k in ("cpu_searches", "gpu_searches", "gpu_failures", "chain_rejects",
                                          "probe_opens", "budget_opens")] + \
                [("cpu_pops", C.c_uint64), ("cpu_s", C.c_double), ("first_conflict_nets", C.c_int32),
-                ("first_conflict_cells", C.c_int32), ("solved_states", C.c_uint64)]
+                ("first_conflict_cells", C.c_int32), ("rescue_tried", C.c_int32), ("rescue_kept", C.c_int32),
+                ("opens_before_rescue", C.c_int32), ("limited", C.c_int32), ("solved_states", C.c_uint64)]
 
 
 SEARCH_CB = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(Query), C.POINTER(Out))
@@ -100,7 +102,10 @@ def library(path=LIB_PATH):
 def spec(kind="c4", seed=1, **kw):
     """Board specs of the configurations: 'c1' a 2-layer 40x30 mm board (6 SOIC-16, 20 0805, 60 nets), 'c4' a
     4-layer 60x50 mm board (2 QFP-64, 8 SOIC-16, 100 or more 0805 so that every one of its 150-300 nets has two pads), 'tiny' a 2-layer
-    18x14 mm board for CPU checks. All on a 50 um grid, 0.2 mm track and clearance, 0.6 mm vias."""
+    18x14 mm board for CPU checks, 'c5' the 8-layer 160x120 mm board (two reserved inner planes, four BGA-256, 10
+    QFP-64, 30 SOIC-16, 400 0805, 1200 nets, classes of 0.1 / 0.15 / 0.25 mm track), 'c5small' the same mix on 50x40 mm
+    (one BGA-256, 160 nets). All on a 50 um grid; one class of 0.2 mm track and clearance with 0.6 mm vias unless
+    stated."""
     s = BoardSpec()
     s.cell_um, s.n_classes, s.edge_mm, s.seed = 50.0, 1, 0.5, seed
     s.track_mm[0], s.clear_mm[0], s.via_mm[0] = 0.2, 0.2, 0.6
@@ -113,6 +118,18 @@ def spec(kind="c4", seed=1, **kw):
         # two pads for every net at least: more 0805s on boards with more nets
         s.n_soic16, s.n_qfp64 = 8, 2
         s.n_0805 = max(100, (2 * s.n_nets + 20 - 8 * 16 - 2 * 64 + 1) // 2)
+    elif kind in ("c5", "c5small"):
+        # 8 layers, the middle two reserved as planes, BGA-256 parts that need fine-pitch escapes (their nets take
+        # the 0.1 mm class), QFP-64 and SOIC-16 parts, 0805 passives, three net classes
+        s.layers, s.n_planes, s.n_classes = 8, 2, 3
+        for k, (t, c, v) in enumerate(((0.1, 0.1, 0.35), (0.15, 0.15, 0.5), (0.25, 0.2, 0.6))):
+            s.track_mm[k], s.clear_mm[k], s.via_mm[k] = t, c, v
+        if kind == "c5":
+            s.width_mm, s.height_mm = 160.0, 120.0
+            s.n_bga256, s.n_qfp64, s.n_soic16, s.n_0805, s.n_nets = 4, 10, 30, 400, 1200
+        else:
+            s.width_mm, s.height_mm = 50.0, 40.0
+            s.n_bga256, s.n_qfp64, s.n_soic16, s.n_0805, s.n_nets = 1, 2, 4, 40, 160
     elif kind == "tiny":
         s.width_mm, s.height_mm, s.layers = 18.0, 14.0, 2
         s.n_soic16, s.n_0805, s.n_qfp64, s.n_nets = 1, 10, 0, 12

@@ -183,3 +183,36 @@ def test_dsn_boards_search_on_the_gpu(tmp_path, board):
         _say(board, "by the reference:", out.stdout.strip())
         nets, conns, opens, viol = (int(x) for x in m.groups())
         assert conns == b.info.n_connections and opens == 0 and viol == 0, out.stdout
+
+
+def test_c5_mix_board_routes_clean_by_the_reference(tmp_path):
+    """A board of the C5 mix on 50 x 40 mm: 8 layers with the middle two reserved as planes (no class routes there;
+    the session's classes say so with use_layer), a BGA-256 at 0.8 mm pitch whose nets take the 0.1 mm class and
+    escape between its pads, QFP-64, SOIC-16 and 0805 parts, three net classes; routed on the GPU in box-disjoint
+    groups with the window stages, legalizing and the rescue pass. No track on a plane layer, the router's own check
+    clean, and the reference's evaluator (where built) counts the same connections and opens and no violation."""
+    import re
+    import subprocess
+    import test_dsn
+    b = R.Board(R.spec("c5small", seed=1))
+    _say("c5 mix", b.info.n_nets, b.info.n_connections)
+    r = R.Router(b, R.default_options(wave=-64))
+    res = r.route()
+    _say({k: res[k] for k in ("opens", "connections", "iterations", "searches", "rescue_tried", "rescue_kept",
+                              "opens_before_rescue", "route_s", "vias")})
+    assert res["violations"] == 0 and res["conflict_nets"] == 0 and r.eval() == (res["opens"], 0)
+    assert res["opens"] <= 0.1 * res["connections"] and res["opens"] <= res["opens_before_rescue"]
+    for n in range(b.info.n_nets):
+        rows = r.net_route(n)
+        assert not np.any((rows[:, 0] == 3) | (rows[:, 0] == 4)), "a track on a reserved plane layer"
+    d, s = str(tmp_path / "c5.dsn"), str(tmp_path / "c5.ses")
+    b.write_dsn(d)
+    assert "use_layer" in open(d).read()
+    r.write_ses(s)
+    if os.path.exists(test_dsn.REF):
+        out = subprocess.run([test_dsn.REF, "-de", d, "--eval", s], capture_output=True, text=True, timeout=300)
+        m = re.search(r"eval: (\d+) nets, (\d+) connections, (\d+) open, (\d+) violations", out.stdout)
+        assert m, out.stdout + out.stderr
+        _say("c5 mix by the reference:", out.stdout.strip())
+        nets, conns, opens, viol = (int(x) for x in m.groups())
+        assert conns == b.info.n_connections and opens == res["opens"] and viol == 0, out.stdout
