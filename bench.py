@@ -174,7 +174,7 @@ def round_workload(b, a):
     for g in gs:
         g.close()
     kern_s = st["kernel_ms"] / 1e3
-    achieved = st["state_sweeps"] * ALGO_BYTES / kern_s / 1e9 if kern_s > 0 else 0.0
+    achieved = st["states"] * ALGO_BYTES / kern_s / 1e9 if kern_s > 0 else 0.0  # 12 B per state per tile visit
     print(json.dumps({
         "metric": "batched speculative round searches/s", "value": round(len(pairs) / t_round, 2),
         "unit": "searches/s", "n_gpus": 1, "higher_is_better": True, "dtype": "int32", "data": "synthetic",
@@ -228,7 +228,7 @@ def c1_workload(b, a):
         "metric": "C1 board route time", "value": round(float(np.median(times)), 4), "unit": "s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(float(np.median(times)) * 1e3, 1),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": "C1: seeded 40x30 mm 2-layer board, 6 SOIC-16 + 20 0805, 50 nets of 2-6 pins, "
+        "config": {"workload": "C1: seeded 40x30 mm 2-layer board, 6 SOIC-16 + 20 0805, 60 nets of 2-6 pins, "
                                "PathFinder, one net at a time", "board_seed": a.board_seed},
         "result": {k: res[k] for k in ("nets", "connections", "opens", "violations", "iterations", "searches",
                                        "length_cells", "vias", "route_hash")},
@@ -344,7 +344,7 @@ def c5board_workload(b, a):
         dsn = os.path.join(tmp, "c5.dsn")
         board.write_dsn(dsn)
         for name, opts in (("plain", dict(wave=0)), ("batched", dict(wave=-64))):
-            r = R.Router(board, R.default_options(time_limit_s=240.0, **opts))
+            r = R.Router(board, R.default_options(time_limit_s=240.0, verbose=1, **opts))  # a line per round (stderr)
             t0 = time.perf_counter()
             res = r.route()
             wall = time.perf_counter() - t0
