@@ -325,6 +325,53 @@ def prior_workload(b, a):
     print(json.dumps(line), flush=True)
 
 
+def c4board_workload(b, a):
+    """C4 boards: held-out seeds routed plain and with the committed learned prior (pcbroute_amd.learn's
+    configurations), every session judged by the reference's own evaluator and every board routed by the reference's
+    CPU router (16 threads) beside them, where oracle/_ref/pcbroute is built. Prints one JSON line with the totals."""
+    import subprocess
+    import tempfile
+    from pcbroute_amd import learn
+    ref = os.path.join(ROOT, "oracle", "_ref", "pcbroute")
+    net = learn.load_net()
+    names = ["plain"] + [c for c in a.configs.split(",") if c and c != "plain"]
+    with tempfile.TemporaryDirectory() as tmp:
+        rows = learn.evaluate(net, [learn.EVAL_SEED0 + k for k in range(a.boards)],
+                              configs={c: learn.CONFIGS[c] for c in names}, session_dir=tmp)
+        tot = {c: {"route_s": 0.0, "opens": 0, "violations": 0, "judge_open": 0, "judge_violations": 0} for c in names}
+        ref_tot = {"route_s": 0.0, "open": 0, "violations": 0}
+        for row in rows:
+            for c in names:
+                x = row[c]
+                tot[c]["route_s"] += x["route_s"]
+                tot[c]["opens"] += x["opens"]
+                tot[c]["violations"] += x["violations"]
+                if os.path.exists(ref):
+                    j = _ref_eval(ref, row["dsn"], x["ses"]) or {}
+                    tot[c]["judge_open"] += j.get("open", 0)
+                    tot[c]["judge_violations"] += j.get("violations", 0)
+            if os.path.exists(ref) and a.reference_router:
+                ses = os.path.join(tmp, f"{row['seed']}_reference.ses")
+                t0 = time.perf_counter()
+                p = subprocess.run([ref, "-de", row["dsn"], "-do", ses, "-mt", "16"], stdout=subprocess.DEVNULL,
+                                   stderr=subprocess.DEVNULL, timeout=3600)
+                ref_tot["route_s"] += time.perf_counter() - t0
+                j = _ref_eval(ref, row["dsn"], ses) if p.returncode == 0 else None
+                ref_tot["open"] += (j or {}).get("open", 0)
+                ref_tot["violations"] += (j or {}).get("violations", 0)
+            log(f"seed {row['seed']} judged")
+    line = {"metric": "board route s, learned vs plain, same opens",
+            "value": round(tot[names[-1]]["route_s"], 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+            "dtype": "int32", "data": "synthetic",
+            "config": {"workload": f"C4: {a.boards} held-out 4-layer 60x50 mm boards (seeds {learn.EVAL_SEED0}..)",
+                       "configs": names},
+            "totals": {c: {k: round(v, 3) for k, v in t.items()} for c, t in tot.items()},
+            "over_plain_time": {c: round(tot[c]["route_s"] / max(1e-9, tot["plain"]["route_s"]), 3) for c in names}}
+    if a.reference_router and os.path.exists(ref):
+        line["reference_cpu_router_16_threads"] = {k: round(v, 3) for k, v in ref_tot.items()}
+    print(json.dumps(line), flush=True)
+
+
 def c5board_workload(b, a):
     """C5: the 8-layer 160x120 mm board (two reserved inner planes, four BGA-256 whose nets escape in the 0.1 mm class,
     10 QFP-64, 30 SOIC-16, 400 0805, 1200 nets, three net classes) routed by the router on the GPU under a 240 s time
@@ -392,13 +439,19 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=("both", "single", "round"), default="both")
-    ap.add_argument("--workload", choices=("kernel", "round", "prior", "c1", "c5", "c5board"), default="kernel",
+    ap.add_argument("--workload", choices=("kernel", "round", "prior", "c1", "c4board", "c5", "c5board"),
+                    default="kernel",
                     help="kernel: the 64 whole-grid searches (the metric); c5: the same on a 3200x2400x8 grid (the "
                          "C5 board's 61.4M states), 16 searches; round: 256 net searches of a speculative "
                          "round on a 2400x1800x6 board with two net classes, as one round and one at a time; "
                          "prior: the corridor CNN's training and inference on full 4-layer boards; c1: the router on "
                          "the C1 board, its session judged by the reference's evaluator, the reference's CPU router "
-                         "timed beside it; c5board: the C5 board routed plain and batched under its time limit")
+                         "timed beside it; c4board: held-out C4 boards, plain and learned (--configs), judged by the "
+                         "reference's evaluator; c5board: the C5 board routed plain and batched under its time limit")
+    ap.add_argument("--boards", type=int, default=8, help="c4board: held-out boards")
+    ap.add_argument("--configs", default="plain,learned", help="c4board: pcbroute_amd.learn configurations")
+    ap.add_argument("--reference-router", action="store_true", help="c4board: also route each board with the "
+                    "reference's CPU router")
     ap.add_argument("--tune", choices=("auto", "off"), default=None,
                     help="auto: the autotuner picks tile rows, local sweeps and delta on a sample drawn with another "
                          "seed (the default for --workload round); off: library defaults or the explicit flags (the "
@@ -440,6 +493,8 @@ def main():
         return c1_workload(b, a)
     if a.workload == "c5board":
         return c5board_workload(b, a)
+    if a.workload == "c4board":
+        return c4board_workload(b, a)
     log("building the board")
     dims = dict(nx=3200, ny=2400, nl=8) if a.workload == "c5" else {}
     if a.workload == "c5" and a.pairs == 64:

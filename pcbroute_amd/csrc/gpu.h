@@ -12,19 +12,19 @@
 #include "../../include/pcbgpu.h"
 
 struct GpuQuery {
-    int nx, ny, NL;                // the full grid (host arrays are [l][j * nx + i])
-    int wi0, wj0, ww, wh;          // search window
-    uint16_t me;                   // the net, as stored in the static grids (0: free, 0xFFFF: blocked)
-    int32_t via_cost;              // < 0: no vias
-    int32_t dirw[8][8];            // per layer, the weight of each of the 8 moves
-    uint8_t lay[8];                // layers a track may use
-    const int32_t* cst[8];         // per layer cost map: >= PCBGPU_INF blocked, negative: some net's own copper
-    const uint16_t* st[8];         // static obstacles for a track centre, per layer
-    const int32_t* vcst;           // the same for a via centre
-    const uint16_t* vst;
-    const uint64_t* tbit;          // target bits over the window states
-    std::vector<std::pair<uint32_t, int32_t>> own_cf;   // window state -> cost factor of the net's own cells
-    std::vector<std::pair<uint32_t, int32_t>> own_vcf;  // window cell -> via factor where the net's own vias are
+    int nx, ny, NL;                // grid width, height and layers; every map is indexed [l][j * nx + i]
+    int wi0, wj0, ww, wh;          // the window searched: origin and size in cells
+    uint16_t me;                   // this net's id in the owner maps (PCBGPU_FREE and PCBGPU_BLOCK are reserved)
+    int32_t via_cost;              // weight of a via; negative when vias are off
+    int32_t dirw[8][8];            // [layer][move]: what a step of each of the 8 moves weighs on that layer
+    uint8_t lay[8];                // 1 where tracks may run on the layer
+    const int32_t* cst[8];         // track cost per layer (PCBGPU_INF and above: no entry; below 0: owned copper)
+    const uint16_t* st[8];         // per layer, which net owns a cell for a track centre (or free / blocked)
+    const int32_t* vcst;           // via cost per cell
+    const uint16_t* vst;           // which net owns a cell for a via centre
+    const uint64_t* tbit;          // one bit per window state: 1 on a target
+    std::vector<std::pair<uint32_t, int32_t>> own_cf;   // (window state, cost factor) where this net has copper
+    std::vector<std::pair<uint32_t, int32_t>> own_vcf;  // (window cell, via factor) where this net has a via
     const uint8_t* prior = nullptr;
     int pi0 = 0, pj0 = 0, pw = 0, ph = 0, pnl = 0;
     const int* pcol = nullptr;
@@ -33,14 +33,14 @@ struct GpuQuery {
     int32_t mlbias[256];
     uint8_t mthr = 0;
     bool corridor = false;
-    std::vector<std::pair<uint32_t, int32_t>> starts;  // window state, initial cost
+    std::vector<std::pair<uint32_t, int32_t>> starts;  // (window state, cost already paid to stand there)
 };
 
-// The solution: costs for the part of the window that was solved (cells outside it read as unreachable).
+// The answer of a solve: the field over the rectangle of the window it covered; states outside read as unreachable.
 struct GpuOut {
     int rx0 = 0, ry0 = 0, rw = 0, rh = 0, NL = 0, ww = 0, wh = 0;
-    int64_t best = -1;                   // the cheapest target reached, as a window state (-1: none)
-    int32_t* d = nullptr;                // pinned host memory from a pool, given back by the destructor
+    int64_t best = -1;                   // window state of the least-cost target, -1 if none was reached
+    int32_t* d = nullptr;                // the field, [l][j * rw + i]; the library's pinned buffer, returned on destruction
     size_t cap = 0;
     GpuOut() = default;
     GpuOut(const GpuOut&) = delete;

@@ -128,82 +128,41 @@ FIELDS = ("opens", "violations", "route_s", "search_s", "prior_s", "upload_s", "
           "first_conflict_cells", "solved_states", "cpu_pops")
 
 
-REF = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "pcbroute")
-
-
-def ref_eval(dsn, ses):
-    """The reference's judge (`pcbroute --eval`) of a session: (connections, open, violations, track mm, vias)."""
-    import re
-    import subprocess
-    out = subprocess.run([REF, "-de", dsn, "--eval", ses], capture_output=True, text=True, timeout=1200)
-    m = re.search(r"eval: \d+ nets, (\d+) connections, (\d+) open, (\d+) violations, ([\d.]+) mm of track, (\d+) vias",
-                  out.stdout)
-    if not m:
-        return None
-    return {"connections": int(m.group(1)), "open": int(m.group(2)), "violations": int(m.group(3)),
-            "track_mm": float(m.group(4)), "vias": int(m.group(5))}
-
-
-def evaluate(net, seeds, configs=CONFIGS, reference=False):
+def evaluate(net, seeds, configs=CONFIGS, session_dir=None):
     """Each held-out board routed under each configuration (one net at a time or batched; plain or learned). With
-    reference (and oracle/_ref/pcbroute built): every configuration's session judged by the reference's --eval, and
-    the board, written as DSN, routed by the reference's own CPU router (16 threads) as one more column."""
-    import subprocess
-    import tempfile
+    session_dir: each board written there as DSN (<seed>.dsn) and each configuration's routes as a session
+    (<seed>_<config>.ses), for an outside judge; their paths go into the rows."""
     rows = []
     for s in seeds:
         b = R.Board(R.spec("c4", seed=s))
         row = {"seed": s, "nets": b.info.n_nets, "connections": b.info.n_connections}
-        with tempfile.TemporaryDirectory() as tmp:
-            dsn = os.path.join(tmp, "board.dsn")
-            if reference:
-                b.write_dsn(dsn)
-            for name, cfg in configs.items():
-                opts = {k: v for k, v in cfg.items() if k != "prior"}
-                r = R.Router(b, R.default_options(**opts))
-                if cfg.get("prior"):
-                    r.set_prior(net)
-                res = r.route()
-                row[name] = {k: res[k] for k in FIELDS}
-                if reference:
-                    ses = os.path.join(tmp, name + ".ses")
-                    r.write_ses(ses)
-                    row[name]["reference_judge"] = ref_eval(dsn, ses)
-                r.close()
-            if reference:
-                threads = min(16, os.cpu_count() or 1)
-                ses = os.path.join(tmp, "reference.ses")
-                t0 = time.time()
-                log(f"seed {s}: the reference's CPU router, {threads} threads")
-                p = subprocess.run([REF, "-de", dsn, "-do", ses, "-mt", str(threads), "-v"], stdout=subprocess.DEVNULL,
-                                   timeout=3600)  # its progress lines go to stderr
-                row["reference_cpu"] = {"route_s": time.time() - t0, "threads": threads,
-                                        "reference_judge": ref_eval(dsn, ses) if p.returncode == 0 else None}
+        if session_dir:
+            row["dsn"] = os.path.join(session_dir, f"{s}.dsn")
+            b.write_dsn(row["dsn"])
+        for name, cfg in configs.items():
+            opts = {k: v for k, v in cfg.items() if k != "prior"}
+            r = R.Router(b, R.default_options(**opts))
+            if cfg.get("prior"):
+                r.set_prior(net)
+            res = r.route()
+            row[name] = {k: res[k] for k in FIELDS}
+            if session_dir:
+                row[name]["ses"] = os.path.join(session_dir, f"{s}_{name}.ses")
+                r.write_ses(row[name]["ses"])
+            r.close()
         rows.append(row)
         log(f"seed {s}: " + "; ".join(f"{k} {row[k]['route_s']:.2f} s {row[k]['opens']} open "
-                                      f"{row[k]['violations']} viol" for k in configs)
-            + (f"; reference cpu {row['reference_cpu']['route_s']:.2f} s {row['reference_cpu']['reference_judge']}"
-               if "reference_cpu" in row else ""))
+                                      f"{row[k]['violations']} viol" for k in configs))
     return rows
 
 
 def summary(rows):
-    names = [k for k in rows[0] if isinstance(rows[0][k], dict) and k != "reference_cpu"]
+    names = [k for k in rows[0] if isinstance(rows[0][k], dict)]
     tot = {k: {"route_s": 0.0, "opens": 0, "violations": 0, "search_fallbacks": 0} for k in names}
     for r in rows:
         for k in names:
             for f in tot[k]:
                 tot[k][f] += r[k][f]
-    if "reference_cpu" in rows[0]:
-        tot["reference_cpu"] = {"route_s": sum(r["reference_cpu"]["route_s"] for r in rows),
-                                "opens": sum((r["reference_cpu"]["reference_judge"] or {}).get("open", 0) for r in rows),
-                                "violations": sum((r["reference_cpu"]["reference_judge"] or {}).get("violations", 0)
-                                                  for r in rows)}
-        for k in names:
-            j = [r[k].get("reference_judge") for r in rows]
-            if all(j):
-                tot[k]["reference_judge_open"] = sum(x["open"] for x in j)
-                tot[k]["reference_judge_violations"] = sum(x["violations"] for x in j)
     for k in names:
         tot[k + "_over_plain_time"] = tot[k]["route_s"] / max(1e-9, tot["plain"]["route_s"])
     return tot
@@ -218,9 +177,6 @@ def main():
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--report", default=None, help="JSON report path")
     ap.add_argument("--load", default=None, help="skip training: evaluate these weights")
-    ap.add_argument("--reference", action="store_true",
-                    help="judge every session with the reference's --eval and route each board with the reference's "
-                         "CPU router too (needs oracle/_ref/pcbroute)")
     ap.add_argument("--configs", default=",".join(CONFIGS),
                     help="comma-separated configurations to evaluate (plain always included)")
     a = ap.parse_args()
@@ -253,8 +209,7 @@ def main():
     bad = [c for c in names if c not in CONFIGS]
     if bad:
         raise SystemExit(f"unknown configurations: {', '.join(bad)} (known: {', '.join(CONFIGS)})")
-    rows = evaluate(net, [EVAL_SEED0 + k for k in range(a.eval)], configs={c: CONFIGS[c] for c in names},
-                    reference=a.reference and os.path.exists(REF))
+    rows = evaluate(net, [EVAL_SEED0 + k for k in range(a.eval)], configs={c: CONFIGS[c] for c in names})
     rep["eval"] = rows
     rep["summary"] = summary(rows)
     rep["total_s"] = time.time() - t0

@@ -8,6 +8,7 @@ from pcbroute_amd import synth
 from pcbroute_amd.abi import INF, Problem
 
 from cases import random_case
+from known_cases import TIES, WIDENING
 
 pytestmark = pytest.mark.gpu
 
@@ -92,17 +93,40 @@ def test_grid_resident_solves(backend, oracle, seed):
         g.close()
 
 
-@pytest.mark.parametrize("seed", range(8))
+# seeds of random_case(seed, 100, 90, 2 + seed % 2) whose targets are reachable (302, 303 and 313 are not)
+BACKTRACE_SEEDS = (300, 301, 304, 305, 306, 307, 308, 309)
+
+
+@pytest.mark.parametrize("seed", BACKTRACE_SEEDS)
 def test_backtrace_matches_oracle(backend, oracle, seed):
-    p = random_case(300 + seed, nx=100, ny=90, nl=2 + seed % 2)
+    p = random_case(seed, nx=100, ny=90, nl=2 + seed % 2)
     _say("backtrace", seed)
     res = backend.solve(p)
-    if res.best < 0:
-        pytest.skip("no target reachable in this case")
+    assert res.best >= 0
     gap, gap2 = seed % 3, float(seed % 4)
     mine = backend.backtrace(p, res, gap, gap2)
     ref = oracle.backtrace(p, res.rect, res.best, res.d, gap, gap2)
     assert mine == ref
+
+
+@pytest.mark.parametrize("case", range(len(WIDENING)))
+def test_widening_known_rectangles(backend, case):
+    """The rectangles, try counts and costs worked out by hand from the widening rule (tests/known_cases.py)."""
+    name, build, (ti, tj), rect, tries, cost = WIDENING[case]
+    p = build()
+    _say("widening", name)
+    res = backend.solve(p)
+    assert (res.rect, res.tries, res.best) == (rect, tries, tj * p.window[2] + ti), name
+    assert res.d[0, tj - rect[1], ti - rect[0]] == cost, name
+
+
+@pytest.mark.parametrize("case", range(len(TIES)))
+def test_backtrace_known_ties(backend, case):
+    name, build = TIES[case]
+    p, chain = build()
+    _say("tie", name)
+    res = backend.solve(p)
+    assert backend.backtrace(p, res) == chain, name
 
 
 def test_unreachable_target(backend, oracle):

@@ -9,7 +9,8 @@ Each check is chosen so that a plausible mistake in the oracle fails it:
   - the reference-order sweep solver (a second algorithm) against Dijkstra;
   - Bellman optimality of every finite state of the canonical field;
   - known answers of the cost resolution: own copper, another net's copper, the prior's bias and corridor;
-  - the backtrace's first-strict-minimum rule, restated naively here on small cases.
+  - the backtrace's first-strict-minimum rule, restated naively here on small cases, and two ties worked by hand;
+  - the widening's rectangles, try counts and costs worked by hand on walls placed on either side of its bound.
 """
 import numpy as np
 import pytest
@@ -19,6 +20,7 @@ from scipy.sparse.csgraph import dijkstra
 from pcbroute_amd.abi import BLOCK, DX, DY, FREE, INF, MAX_LAYERS, Problem, default_dirw
 
 from cases import random_case
+from known_cases import TIES, WIDENING
 
 
 def empty(nl, ny, nx, cost=1):
@@ -327,3 +329,25 @@ def test_widening_keeps_the_optimum(oracle):
         l, c = divmod(best, 600 * 500)
         jj, ii = divmod(c, 600)
         assert res["d"][l, jj - ry0, ii - rx0] == opt
+
+
+@pytest.mark.parametrize("case", range(len(WIDENING)))
+def test_widening_known_rectangles(oracle, case):
+    """The rectangles, try counts and costs worked out by hand from the widening rule (tests/known_cases.py)."""
+    name, build, (ti, tj), rect, tries, cost = WIDENING[case]
+    p = build()
+    res = oracle.solve(p)
+    assert res["rect"] == rect, name
+    assert res["tries"] == tries, name
+    assert res["best"] == tj * p.window[2] + ti, name
+    assert res["d"][0, tj - rect[1], ti - rect[0]] == cost, name
+
+
+@pytest.mark.parametrize("case", range(len(TIES)))
+def test_backtrace_known_ties(oracle, case):
+    """Ties between moves, and between a move and a via, broken as worked out by hand (tests/known_cases.py)."""
+    name, build = TIES[case]
+    p, chain = build()
+    res = oracle.solve(p)
+    assert res["best"] == chain[0][0]
+    assert oracle.backtrace(p, res["rect"], res["best"], res["d"]) == chain, name
