@@ -70,8 +70,8 @@ typedef struct pcbprior_stats {
 void pcbprior_stats_enable(pcbprior* p, int on);
 void pcbprior_stats_read(pcbprior* p, pcbprior_stats* s, int reset);
 
-/* One 16x16x16 BF16 matrix-core product, for the lane-layout test: a (16x16), b (16x16) row-major floats rounded
- * to BF16, d (16x16) = a @ b in FP32. */
+/* One 16x16x32 BF16 matrix-core product (the shape the conv kernels use), for the lane-layout test: a (16x32),
+ * b (32x16) row-major floats rounded to BF16, d (16x16) = a @ b in FP32. */
 int pcbprior_mfma_check(const float* a, const float* b, float* d);
 
 const char* pcbprior_last_error(void);

@@ -1,4 +1,4 @@
-// prior_cnn.hip - the learned corridor prior: conv3x3 layers on BF16 matrix cores (MFMA 16x16x16, FP32
+// prior_cnn.hip - the learned corridor prior: conv3x3 layers on gfx950's BF16 matrix cores (MFMA 16x16x32, FP32
 // accumulation), forward, backward and Adam, behind include/pcbprior.h.
 //
 // Layout. Activations are pixel-major BF16 with 32 channel slots (NHWC, padded with zeros). A layer's weights are
@@ -6,12 +6,12 @@
 // forward one in that layout and a flipped one wf[ci][k'] = w[co][(8 - d) * 32 + ci] with k' = d * 32 + co, which
 // makes the gradient with respect to a layer's input the same implicit GEMM as a forward pass.
 //
-// Implicit GEMM. A wavefront computes 16 consecutive pixels of a row times 32 output channels: two 16x16 tiles,
-// K = 9 positions x 32 input channels in 18 steps of 16. Operand fragments (MFMA 16x16x16): lane l holds A[l % 16][4
-// (l / 16) + j] and B[4 (l / 16) + j][l % 16] for j = 0..3, and gets D[4 (l / 16) + j][l % 16]; a lane's 4 A values
-// are 4 consecutive input channels of one pixel (8 bytes), its 4 B values 4 consecutive k of one output channel.
+// Matrix-core fragments (v_mfma_f32_16x16x32_bf16, D = A B + C with A 16 x 32, B 32 x 16): lane l holds A[l % 16][8
+// (l / 16) + j] and B[8 (l / 16) + j][l % 16] for j = 0..7 (16 bytes each), and gets D[4 (l / 16) + j][l % 16] for
+// j = 0..3. One 3x3 tap of 32 channels is exactly one K step of 32.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -41,8 +41,8 @@ constexpr int C = PCBPRIOR_MAX_CH;  // channel slots of an activation
 constexpr int KK = 9 * C;           // the K of a layer's implicit GEMM
 constexpr int PX = 64;              // pixels of a row a workgroup takes (4 wavefronts x 16)
 
-typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 // BF16 from FP32: round to nearest, ties to even (the IEEE rule); NaN stays a quiet NaN
 __host__ __device__ inline uint16_t f2bf(float f) {
@@ -59,13 +59,9 @@ __host__ __device__ inline float bf2f(uint16_t h) {
     return f;
 }
 
-__device__ inline v4f mfma16(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
-
-__device__ inline v4s load4(const uint16_t* p) {  // 4 consecutive BF16 values (8-byte aligned)
-    const uint2 u = *reinterpret_cast<const uint2*>(p);
-    v4s r;
-    r[0] = short(u.x & 0xFFFF); r[1] = short(u.x >> 16); r[2] = short(u.y & 0xFFFF); r[3] = short(u.y >> 16);
-    return r;
+// D += A B on the matrix cores, the fragments as 16-byte vectors of 8 BF16 values
+__device__ inline v4f mfma32(uint4 a, uint4 b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
 }
 
 // ---------------------------------------------------------------- kernels
@@ -78,154 +74,175 @@ __global__ void k_to_bf16(const float* x, uint16_t* out, size_t npix, int cin) {
     out[i] = c < cin ? f2bf(x[p * cin + c]) : uint16_t(0);
 }
 
-// y = conv3x3(x, wt) + bias, the implicit GEMM above. A block takes 64 pixels of CR (8) consecutive rows, one row per
-// wavefront, all 32 output channels. It stages the (CR + 2) x 66-pixel input patch and the layer's weights in LDS
-// once; a wavefront then holds 4 x 2 accumulator tiles (64 pixels x 32 channels) and per K step reads 4 A and 2 B
-// fragments from LDS for 8 matrix-core products. The results go through LDS so that each lane finishes one pixel:
-// bias, ReLU, the mask (zero where the mask activation is zero: the derivative of the layer before's ReLU), then
-// coalesced BF16 stores of its 32 slots, or FP32 logits of the first cout channels. With `valid` (rows, columns per
-// image) the outputs outside image n's valid extent are 0, so an image padded into a larger batch is computed exactly
-// as alone at its valid size (whose conv reads zeros past its edges).
-constexpr int CR = 8;         // rows per block
+// y = conv3x3(x, wt) + bias as an implicit GEMM with the weights as A (rows: output channels, K: tap x input
+// channel) and the input as B (K: one tap's 32 input channels of a pixel, columns: pixels), so that a lane's result
+// is 4 consecutive output channels of one pixel and goes straight to memory. A tile is 64 pixels of CR (8)
+// consecutive rows, one row per wavefront, all 32 output channels; tiles are numbered column block first, then row
+// block, then image. A block is persistent: it stages the layer's weights once, then takes tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ... (the blocks running at once cover neighbouring tiles, which measured faster than each
+// block walking its own strip); while a tile's products run, the next tile's (CR + 2) x 66-pixel input patch is
+// already on its way into registers, and goes to LDS between two barriers once every wavefront is done with the
+// current one. Per tap a wavefront reads 2 A and 4 B fragments (16 bytes each) from LDS for 8 matrix-core products
+// into 4 x 2 accumulator tiles (64 pixels x 32 channels). Each lane then finishes its 4 channels of 8 pixel-tiles:
+// bias, ReLU, the mask (zero where the mask activation is zero: the derivative of the layer before's ReLU), BF16
+// stores of 8 bytes per pixel, or FP32 logits of the first cout channels. With `valid` (rows, columns per image) the
+// outputs outside image n's valid extent are 0, so an image padded into a larger batch is computed exactly as alone
+// at its valid size (whose conv reads zeros past its edges).
+constexpr int CR = 8;         // rows per tile
 constexpr int NT = 64 * CR;   // threads per block
-constexpr int XSP = 36;       // LDS pitch of a staged input pixel (BF16): 18 dwords, conflict-free 8-byte reads
-constexpr int WSP = KK + 8;   // LDS pitch of a staged weight row
-constexpr int OSP = C + 1;    // LDS pitch of an output pixel (FP32)
+constexpr int XSP = 40;       // LDS pitch of a staged input pixel (BF16): 80 bytes, conflict-free 16-byte reads
+constexpr int WSP = KK + 8;   // LDS pitch of a staged weight row (BF16): 592 bytes, conflict-free 16-byte reads
 constexpr int XS_N = (CR + 2) * (PX + 2) * XSP, WS_N = C * WSP;  // BF16 elements
-constexpr int SM_N = XS_N + WS_N > CR * PX * OSP * 2 ? XS_N + WS_N : CR * PX * OSP * 2;  // staging, then output
+constexpr int CONV_BLOCKS_PER_CU = 2;  // what the block's LDS allows
 __global__ __launch_bounds__(NT) void k_conv(const uint16_t* x, const uint16_t* wt, const float* bias, uint16_t* ybf,
-                                              float* yf, const uint16_t* mask, int relu, int cout, int H, int W,
+                                              float* yf, const uint16_t* mask, int relu, int cout, int N, int H, int W,
                                               const int32_t* valid) {
-    __shared__ __align__(16) uint16_t sm[SM_N];
+    __shared__ __align__(16) uint16_t sm[XS_N + WS_N];
     uint16_t* XS = sm;
     uint16_t* WS = sm + XS_N;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int n = blockIdx.z, y0 = blockIdx.y * CR, x0 = blockIdx.x * PX;
-    const size_t img = size_t(n) * H * W;
-    // Staging: every thread issues all its loads (weights: 8 BF16 each; the input patch, zero outside the image)
-    // before its first LDS store, so the block waits for one round trip, not one per load.
+    const int tiles_x = W / PX, tiles_y = (H + CR - 1) / CR, ntiles = tiles_x * tiles_y * N;
     constexpr int NW = C * KK / 8, NX = (CR + 2) * (PX + 2) * (C / 8);
     constexpr int WIT = (NW + NT - 1) / NT, XIT = (NX + NT - 1) / NT;
-    uint4 wv[WIT], xv[XIT];
+    uint4 xv[XIT];
+    // a tile's input patch into registers (zero outside the image), then into LDS
+    auto load_patch = [&](int t) {
+        const int x0 = (t % tiles_x) * PX, y0 = (t / tiles_x % tiles_y) * CR;
+        const size_t img = size_t(t / (tiles_x * tiles_y)) * H * W;
 #pragma unroll
-    for (int it = 0; it < WIT; ++it) {
-        const int e = tid + it * NT;
-        wv[it] = make_uint4(0, 0, 0, 0);
-        if (e < NW) wv[it] = *reinterpret_cast<const uint4*>(wt + size_t(e / (KK / 8)) * KK + (e % (KK / 8)) * 8);
-    }
-#pragma unroll
-    for (int it = 0; it < XIT; ++it) {
-        const int e = tid + it * NT;
-        const int r = e / ((PX + 2) * (C / 8)), rem = e % ((PX + 2) * (C / 8)), q = rem / (C / 8), c8 = rem % (C / 8);
-        const int py = y0 - 1 + r, px = x0 - 1 + q;
-        xv[it] = make_uint4(0, 0, 0, 0);
-        if (e < NX && py >= 0 && py < H && px >= 0 && px < W)
-            xv[it] = *reinterpret_cast<const uint4*>(x + (img + size_t(py) * W + px) * C + c8 * 8);
-    }
-#pragma unroll
-    for (int it = 0; it < WIT; ++it) {
-        const int e = tid + it * NT;
-        if (e < NW) *reinterpret_cast<uint4*>(WS + (e / (KK / 8)) * WSP + (e % (KK / 8)) * 8) = wv[it];
-    }
-#pragma unroll
-    for (int it = 0; it < XIT; ++it) {
-        const int e = tid + it * NT;
-        if (e < NX) {
+        for (int it = 0; it < XIT; ++it) {
+            const int e = tid + it * NT;
             const int r = e / ((PX + 2) * (C / 8)), rem = e % ((PX + 2) * (C / 8)), q = rem / (C / 8), c8 = rem % (C / 8);
-            uint2* d = reinterpret_cast<uint2*>(XS + (r * (PX + 2) + q) * XSP + c8 * 8);
-            d[0] = make_uint2(xv[it].x, xv[it].y);
-            d[1] = make_uint2(xv[it].z, xv[it].w);
+            const int py = y0 - 1 + r, px = x0 - 1 + q;
+            xv[it] = make_uint4(0, 0, 0, 0);
+            if (e < NX && py >= 0 && py < H && px >= 0 && px < W)
+                xv[it] = *reinterpret_cast<const uint4*>(x + (img + size_t(py) * W + px) * C + c8 * 8);
         }
+    };
+    auto store_patch = [&]() {
+#pragma unroll
+        for (int it = 0; it < XIT; ++it) {
+            const int e = tid + it * NT;
+            if (e < NX) {
+                const int r = e / ((PX + 2) * (C / 8)), rem = e % ((PX + 2) * (C / 8)), q = rem / (C / 8), c8 = rem % (C / 8);
+                *reinterpret_cast<uint4*>(XS + (r * (PX + 2) + q) * XSP + c8 * 8) = xv[it];
+            }
+        }
+    };
+    {  // the weights and the first tile: every load issued before the first LDS store
+        uint4 wv[WIT];
+#pragma unroll
+        for (int it = 0; it < WIT; ++it) {
+            const int e = tid + it * NT;
+            wv[it] = make_uint4(0, 0, 0, 0);
+            if (e < NW) wv[it] = *reinterpret_cast<const uint4*>(wt + size_t(e / (KK / 8)) * KK + (e % (KK / 8)) * 8);
+        }
+        if (int(blockIdx.x) < ntiles) load_patch(blockIdx.x);
+#pragma unroll
+        for (int it = 0; it < WIT; ++it) {
+            const int e = tid + it * NT;
+            if (e < NW) *reinterpret_cast<uint4*>(WS + (e / (KK / 8)) * WSP + (e % (KK / 8)) * 8) = wv[it];
+        }
+        store_patch();
     }
     __syncthreads();
     const int pr = lane & 15, kq = lane >> 4;
-    v4f acc[4][2];
+    // lane (pr, kq) finishes channels 16 t + 4 kq + j of its pixels
+    float bv[2][4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) acc[m][0] = acc[m][1] = v4f{0, 0, 0, 0};
-    for (int kk = 0; kk < KK / 16; ++kk) {
-        const int d = kk >> 1, dy = d / 3, dx = d % 3, ci = (kk & 1) * 16 + kq * 4;
-        const v4s b0 = load4(WS + pr * WSP + kk * 16 + kq * 4);
-        const v4s b1 = load4(WS + (16 + pr) * WSP + kk * 16 + kq * 4);
-        const uint16_t* xr = XS + ((wave + dy) * (PX + 2) + pr + dx) * XSP + ci;
+    for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const v4s a = load4(xr + m * 16 * XSP);
-            acc[m][0] = mfma16(a, b0, acc[m][0]);
-            acc[m][1] = mfma16(a, b1, acc[m][1]);
-        }
-    }
-    __syncthreads();  // the staging space becomes the output tile
-    float* O = reinterpret_cast<float*>(sm) + wave * PX * OSP;
+        for (int j = 0; j < 4; ++j) bv[t][j] = bias ? bias[16 * t + 4 * kq + j] : 0.f;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int tn = t + gridDim.x;
+        if (tn < ntiles) load_patch(tn);  // in flight during this tile's products
+        v4f acc[4][2];
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+        for (int m = 0; m < 4; ++m) acc[m][0] = acc[m][1] = v4f{0, 0, 0, 0};
+#pragma unroll 3
+        for (int d = 0; d < 9; ++d) {  // tap (dy, dx): K = its 32 input channels, lane quarter kq takes 8 of them
+            const int dy = d / 3, dx = d % 3;
+            const uint4 a0 = *reinterpret_cast<const uint4*>(WS + pr * WSP + d * C + kq * 8);
+            const uint4 a1 = *reinterpret_cast<const uint4*>(WS + (16 + pr) * WSP + d * C + kq * 8);
+            const uint16_t* xr = XS + ((wave + dy) * (PX + 2) + pr + dx) * XSP + kq * 8;
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) O[(m * 16 + kq * 4 + j) * OSP + 16 * t + pr] = acc[m][t][j];
-    __syncthreads();
-    const int y = y0 + wave;
-    if (y >= H) return;
-    const size_t p = img + size_t(y) * W + x0 + lane;  // W is a multiple of 64: every lane's pixel is inside
-    const float* o = O + lane * OSP;
-    const bool inside = !valid || (y < valid[2 * n] && x0 + lane < valid[2 * n + 1]);
-    if (yf) {
-        for (int co = 0; co < cout; ++co) {
-            float v = o[co] + (bias ? bias[co] : 0.f);
-            yf[p * cout + co] = inside ? (relu ? fmaxf(v, 0.f) : v) : 0.f;
-        }
-    }
-    if (ybf) {
-        uint4 mk[C / 8];
-        if (mask)
-#pragma unroll
-            for (int q = 0; q < C / 8; ++q) mk[q] = reinterpret_cast<const uint4*>(mask + p * C)[q];
-#pragma unroll
-        for (int q = 0; q < C / 8; ++q) {
-            uint32_t w[4];
-#pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                uint32_t pair = 0;
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const int co = q * 8 + h * 2 + k;
-                    float v = o[co] + (bias ? bias[co] : 0.f);
-                    if (relu) v = fmaxf(v, 0.f);
-                    if (mask) {
-                        const uint32_t mw = (&mk[q].x)[h];
-                        if (((mw >> (16 * k)) & 0x7FFF) == 0) v = 0.f;  // the ReLU was off there
-                    }
-                    if (!inside) v = 0.f;
-                    pair |= uint32_t(f2bf(v)) << (16 * k);
-                }
-                w[h] = pair;
+            for (int m = 0; m < 4; ++m) {
+                const uint4 b = *reinterpret_cast<const uint4*>(xr + m * 16 * XSP);
+                acc[m][0] = mfma32(a0, b, acc[m][0]);
+                acc[m][1] = mfma32(a1, b, acc[m][1]);
             }
-            reinterpret_cast<uint4*>(ybf + p * C)[q] = make_uint4(w[0], w[1], w[2], w[3]);
         }
+        const int n = t / (tiles_x * tiles_y), x0 = (t % tiles_x) * PX, y = (t / tiles_x % tiles_y) * CR + wave;
+        if (y < H) {
+            const size_t img = size_t(n) * H * W;
+            const bool row_in = !valid || y < valid[2 * n];
+            const int cols = valid ? valid[2 * n + 1] : W;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int px = x0 + 16 * m + pr;  // W is a multiple of 64: every pixel is inside
+                const size_t p = img + size_t(y) * W + px;
+                const bool inside = row_in && px < cols;
+#pragma unroll
+                for (int tt = 0; tt < 2; ++tt) {
+                    const int c0 = 16 * tt + 4 * kq;
+                    float v[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        v[j] = acc[m][tt][j] + bv[tt][j];
+                        if (relu) v[j] = fmaxf(v[j], 0.f);
+                        if (!inside) v[j] = 0.f;
+                    }
+                    if (yf) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (c0 + j < cout) yf[p * cout + c0 + j] = v[j];
+                    }
+                    if (ybf) {
+                        if (mask) {
+                            const uint2 mk = *reinterpret_cast<const uint2*>(mask + p * C + c0);
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)  // zero where the ReLU was off
+                                if ((((j < 2 ? mk.x : mk.y) >> (16 * (j & 1))) & 0x7FFF) == 0) v[j] = 0.f;
+                        }
+                        *reinterpret_cast<uint2*>(ybf + p * C + c0) =
+                            make_uint2(uint32_t(f2bf(v[0])) | uint32_t(f2bf(v[1])) << 16,
+                                       uint32_t(f2bf(v[2])) | uint32_t(f2bf(v[3])) << 16);
+                    }
+                }
+            }
+        }
+        __syncthreads();  // every wavefront is done with this tile's patch
+        if (tn < ntiles) store_patch();
+        __syncthreads();
     }
 }
 
 // dW[co][k] += sum over the block's pixels of dy[p][co] * x_im2col[p][k]; db[co] += sum of dy[p][co]. The reduction
-// runs over pixels, so a lane's 4 operand values are 4 consecutive pixels of one channel: the block stages its output
-// gradient and its input rows transposed in LDS (a channel's pixels contiguous), and a fragment is one aligned 8-byte
-// read (two, and a byte shift, where the 3x3 tap's column offset misaligns it). A block takes 64 columns of WG_ROWS
-// rows in chunks of WR rows (WR + 2 input rows staged per chunk); wavefront w accumulates the k tiles w, w + 4, ...
-// against both co tiles in registers and adds them into dW with FP32 atomics once.
-constexpr int WG_ROWS = 16, WR = 4;
-constexpr int DTP = WR * PX + 4;         // LDS pitch of a channel of the staged output gradient (BF16)
-constexpr int XTW = PX + 8;              // a staged input row: columns xb - 4 .. xb + 67, column xb + c at c + 4
-constexpr int XTP = (WR + 2) * XTW + 4;  // LDS pitch of a channel of the staged input
-__device__ inline v4s ld_px4(const uint16_t* row, int pos, int sh) {  // row[pos .. pos + 3]; sh = pos & 3, uniform
-    const uint2 lo = *reinterpret_cast<const uint2*>(row + (pos & ~3));
-    uint32_t r0 = lo.x, r1 = lo.y;
-    if (sh) {
-        const uint2 hi = *reinterpret_cast<const uint2*>(row + (pos & ~3) + 4);
-        if (sh == 1) { r0 = __builtin_amdgcn_alignbyte(lo.y, lo.x, 2); r1 = __builtin_amdgcn_alignbyte(hi.x, lo.y, 2); }
-        else if (sh == 2) { r0 = lo.y; r1 = hi.x; }
-        else { r0 = __builtin_amdgcn_alignbyte(hi.x, lo.y, 2); r1 = __builtin_amdgcn_alignbyte(hi.y, hi.x, 2); }
-    }
-    v4s v;
-    v[0] = short(r0 & 0xFFFF); v[1] = short(r0 >> 16); v[2] = short(r1 & 0xFFFF); v[3] = short(r1 >> 16);
-    return v;
+// runs over pixels, so a lane's 8 operand values are 8 consecutive pixels of one channel: the output gradient and the
+// input rows are staged transposed in LDS (a channel's pixels contiguous), and a fragment is one aligned 16-byte read
+// (two, and a shift, where the 3x3 tap's column offset misaligns it). A work item is WR rows of 64 columns of one
+// image (WR + 2 input rows staged); items are numbered row chunk first, then column block, then image. A block is
+// persistent and holds two groups of 4 wavefronts with a staging area each; it takes a contiguous run of items
+// (mostly one column strip downwards, so the input rows an item shares with the one before are fresh in L2), the
+// groups taking alternate items, each with its next item's loads in flight during the current item's products.
+// Wavefront w of a group accumulates the k tiles w, w + 4, ... against both co tiles in registers over all its
+// items; at the end the second group's sums are added to the first's in LDS and the first adds them into dW with
+// FP32 atomics, once per block.
+constexpr int WR = 4;
+constexpr int DTP = WR * PX + 8;         // LDS pitch of a channel of the staged output gradient (BF16): 16 * odd bytes
+constexpr int XTW = PX + 16;             // a staged input row: column xb + c at c + 8 (columns xb - 1 .. xb + 64 used)
+constexpr int XTP = (WR + 2) * XTW + 8;  // LDS pitch of a channel of the staged input (BF16): 16 * odd bytes
+constexpr int WG_NT = 512;               // threads per block: two groups of 256
+constexpr int WGRAD_BLOCKS_PER_CU = 1;   // what the block's LDS allows
+// row[pos .. pos + 7] as a fragment; sh = pos & 7 is 0, 1 or 7 (the three tap columns) and uniform across the
+// wavefront: one aligned 16-byte read, or two and a shift by one BF16 value up or down
+__device__ inline uint32_t pair_at(uint32_t lo, uint32_t hi) { return __builtin_amdgcn_alignbyte(hi, lo, 2); }
+__device__ inline uint4 ld_px8(const uint16_t* row, int pos, int sh) {
+    const uint4 lo = *reinterpret_cast<const uint4*>(row + (pos & ~7));
+    if (sh == 0) return lo;
+    const uint4 hi = *reinterpret_cast<const uint4*>(row + (pos & ~7) + 8);
+    if (sh == 1) return make_uint4(pair_at(lo.x, lo.y), pair_at(lo.y, lo.z), pair_at(lo.z, lo.w), pair_at(lo.w, hi.x));
+    return make_uint4(pair_at(lo.w, hi.x), pair_at(hi.x, hi.y), pair_at(hi.y, hi.z), pair_at(hi.z, hi.w));  // sh == 7
 }
 // 4 pixels x 8 channels (four 16-byte loads, pixel-major) into 8 runs of 4 pixels of one channel (8-byte stores)
 __device__ inline void put4(uint16_t* dst, int pitch, const uint4 (&v)[4]) {
@@ -237,103 +254,153 @@ __device__ inline void put4(uint16_t* dst, int pitch, const uint4 (&v)[4]) {
         *reinterpret_cast<uint2*>(dst + k * pitch) = make_uint2(e[0] | (e[1] << 16), e[2] | (e[3] << 16));
     }
 }
-__global__ __launch_bounds__(256) void k_wgrad(const uint16_t* x, const uint16_t* dy, float* dw, float* db, int H, int W) {
-    __shared__ __align__(16) uint16_t DT[C * DTP];
-    __shared__ __align__(16) uint16_t XT[C * XTP];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int n = blockIdx.z, xb = blockIdx.x * PX;
+__global__ __launch_bounds__(WG_NT) void k_wgrad(const uint16_t* x, const uint16_t* dy, float* dw, float* db, int N,
+                                                  int H, int W) {
+    __shared__ __align__(16) uint16_t DTs[2][C * DTP];
+    __shared__ __align__(16) uint16_t XTs[2][C * XTP];
+    const int grp = threadIdx.x >> 8, tid = threadIdx.x & 255, lane = tid & 63, wave = tid >> 6;
+    uint16_t* DT = DTs[grp];
+    uint16_t* XT = XTs[grp];
     const int pr = lane & 15, kq = lane >> 4;
-    const size_t img = size_t(n) * H * W;
+    const int cols_x = W / PX, chunks = (H + WR - 1) / WR, nitems = cols_x * chunks * N;
+    const int per = (nitems + gridDim.x - 1) / gridDim.x, it_beg = int(blockIdx.x) * per;
+    const int it_end = min(nitems, it_beg + per);
     constexpr int KT = KK / 16;  // 18 k tiles: wavefront w takes w, w + 4, ..., at most 5 of them
+    constexpr int NDT = WR * (PX / 4) * (C / 8), DIT = (NDT + 255) / 256;
+    constexpr int NXT = (WR + 2) * ((PX + 8) / 4) * (C / 8), XIT = (NXT + 255) / 256;
+    uint4 vd[DIT][4], vx[XIT][4];
+    // an item's output gradient rows c0 .. c0 + WR - 1 (zero past the image) and input rows c0 - 1 .. c0 + WR in
+    // groups of 4 columns xb - 4 + 4 g .. xb - 1 + 4 g, g = 0..17 (zero outside the image)
+    auto load_item = [&](int it0) {
+        const int xb = (it0 / chunks % cols_x) * PX, c0 = (it0 % chunks) * WR;
+        const size_t img = size_t(it0 / (cols_x * chunks)) * H * W;
+#pragma unroll
+        for (int it = 0; it < DIT; ++it) {
+            const int e = tid + it * 256;
+            const int r = e / ((PX / 4) * (C / 8)), rem = e % ((PX / 4) * (C / 8)), g = rem / (C / 8), c8 = rem % (C / 8);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                vd[it][j] = (e < NDT && c0 + r < H)
+                                ? *reinterpret_cast<const uint4*>(dy + (img + size_t(c0 + r) * W + xb + 4 * g + j) * C + c8 * 8)
+                                : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int it = 0; it < XIT; ++it) {
+            const int e = tid + it * 256;
+            const int r = e / (((PX + 8) / 4) * (C / 8)), rem = e % (((PX + 8) / 4) * (C / 8)), g = rem / (C / 8),
+                      c8 = rem % (C / 8);
+            const int py = c0 - 1 + r;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int px = xb - 4 + 4 * g + j;
+                vx[it][j] = (e < NXT && py >= 0 && py < H && px >= 0 && px < W)
+                                ? *reinterpret_cast<const uint4*>(x + (img + size_t(py) * W + px) * C + c8 * 8)
+                                : make_uint4(0, 0, 0, 0);
+            }
+        }
+    };
+    auto store_item = [&]() {
+#pragma unroll
+        for (int it = 0; it < DIT; ++it) {
+            const int e = tid + it * 256;
+            const int r = e / ((PX / 4) * (C / 8)), rem = e % ((PX / 4) * (C / 8)), g = rem / (C / 8), c8 = rem % (C / 8);
+            if (e < NDT) put4(DT + (c8 * 8) * DTP + r * PX + 4 * g, DTP, vd[it]);
+        }
+#pragma unroll
+        for (int it = 0; it < XIT; ++it) {
+            const int e = tid + it * 256;
+            const int r = e / (((PX + 8) / 4) * (C / 8)), rem = e % (((PX + 8) / 4) * (C / 8)), g = rem / (C / 8),
+                      c8 = rem % (C / 8);
+            if (e < NXT) put4(XT + (c8 * 8) * XTP + r * XTW + 4 + 4 * g, XTP, vx[it]);  // column xb + c at c + 8
+        }
+    };
     v4f acc[2][5];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int u = 0; u < 5; ++u) acc[t][u] = v4f{0, 0, 0, 0};
     float bsum = 0.f;
-    const int y1 = min(H, (blockIdx.y + 1) * WG_ROWS);
-    for (int c0 = blockIdx.y * WG_ROWS; c0 < y1; c0 += WR) {
-        const int rows = min(WR, y1 - c0);
-        for (int e = tid; e < WR * (PX / 4) * (C / 8); e += 256) {  // output gradient rows c0 .. c0 + WR - 1
-            const int r = e / ((PX / 4) * (C / 8)), rem = e % ((PX / 4) * (C / 8)), g = rem / (C / 8), c8 = rem % (C / 8);
-            uint4 v[4];
+    // group grp takes items it_beg + grp, it_beg + grp + 2, ...; both groups run the same number of rounds (barriers)
+    const int rounds = (max(0, it_end - it_beg) + 1) / 2;
+    if (it_beg + grp < it_end) {
+        load_item(it_beg + grp);
+        store_item();
+    }
+    __syncthreads();
+    for (int k = 0; k < rounds; ++k) {
+        const int it0 = it_beg + grp + 2 * k, itn = it0 + 2;
+        if (itn < it_end) load_item(itn);  // in flight during this item's products
+        if (it0 < it_end) {
+            const int rows = min(WR, H - (it0 % chunks) * WR);
+            {  // the bias gradient: thread t sums 32 pixels of channel t % 32 (the partial sums meet at the end)
+                const int ch = tid & (C - 1), q0 = (tid / C) * 32;
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                v[j] = r < rows ? *reinterpret_cast<const uint4*>(dy + (img + size_t(c0 + r) * W + xb + 4 * g + j) * C + c8 * 8)
-                                : make_uint4(0, 0, 0, 0);
-            put4(DT + (c8 * 8) * DTP + r * PX + 4 * g, DTP, v);
-        }
-        {  // input rows c0 - 1 .. c0 + WR: every load of the thread issued before its first LDS store
-            constexpr int NXT = (WR + 2) * (XTW / 4) * (C / 8), XIT = (NXT + 255) / 256;
-            uint4 v[XIT][4];
+                for (int q = q0; q < q0 + 32; q += 8)
+                    if (q < rows * PX) {
+                        const uint4 d8 = *reinterpret_cast<const uint4*>(DT + ch * DTP + q);
 #pragma unroll
-            for (int it = 0; it < XIT; ++it) {
-                const int e = tid + it * 256;
-                const int r = e / ((XTW / 4) * (C / 8)), rem = e % ((XTW / 4) * (C / 8)), g = rem / (C / 8), c8 = rem % (C / 8);
-                const int py = c0 - 1 + r;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int px = xb - 4 + 4 * g + j;
-                    v[it][j] = (e < NXT && py >= 0 && py < H && px >= 0 && px < W)
-                                   ? *reinterpret_cast<const uint4*>(x + (img + size_t(py) * W + px) * C + c8 * 8)
-                                   : make_uint4(0, 0, 0, 0);
-                }
+                        for (int j = 0; j < 4; ++j) {
+                            const uint32_t u = (&d8.x)[j];
+                            bsum += bf2f(uint16_t(u & 0xFFFF)) + bf2f(uint16_t(u >> 16));
+                        }
+                    }
             }
+            for (int r = 0; r < rows; ++r)
 #pragma unroll
-            for (int it = 0; it < XIT; ++it) {
-                const int e = tid + it * 256;
-                const int r = e / ((XTW / 4) * (C / 8)), rem = e % ((XTW / 4) * (C / 8)), g = rem / (C / 8), c8 = rem % (C / 8);
-                if (e < NXT) put4(XT + (c8 * 8) * XTP + r * XTW + 4 * g, XTP, v[it]);
-            }
-        }
-        __syncthreads();
-        {  // the bias gradient: thread t sums 32 pixels of channel t % 32 (the 8 partial sums meet at the end)
-            const int ch = tid & (C - 1), q0 = (tid / C) * 32;
+                for (int ps = 0; ps < PX / 32; ++ps) {
+                    // A = dy^T (row co, column pixel), B = x_im2col (row pixel, column k); K = 32 pixels of row r
+                    const int p0 = ps * 32 + kq * 8;
+                    const uint4 a0 = *reinterpret_cast<const uint4*>(DT + pr * DTP + r * PX + p0);
+                    const uint4 a1 = *reinterpret_cast<const uint4*>(DT + (16 + pr) * DTP + r * PX + p0);
 #pragma unroll
-            for (int q = q0; q < q0 + 32; q += 4)
-                if (q < rows * PX) {
-                    const v4s d4 = ld_px4(DT + ch * DTP, q, 0);
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) bsum += bf2f(uint16_t(d4[j]));
-                }
-        }
-        for (int r = 0; r < rows; ++r)
-#pragma unroll
-            for (int ps = 0; ps < PX / 16; ++ps) {
-                // A = dy^T (row co, column pixel), B = x_im2col (row pixel, column k)
-                const int p0 = ps * 16 + kq * 4;
-                const v4s a0 = ld_px4(DT + pr * DTP + r * PX, p0, 0), a1 = ld_px4(DT + (16 + pr) * DTP + r * PX, p0, 0);
-#pragma unroll
-                for (int u = 0; u < 5; ++u) {
-                    const int kt = wave + 4 * u;
-                    if (kt < KT) {
-                        // k = kt * 16 + pr: tap d = k / C (the same for the whole k tile), channel ci = k % C
-                        const int d = kt >> 1, ci = (kt & 1) * 16 + pr, dyy = d / 3, dxx = d % 3;
-                        const v4s bb = ld_px4(XT + ci * XTP + (r + dyy) * XTW, p0 + dxx + 3, (dxx + 3) & 3);
-                        acc[0][u] = mfma16(a0, bb, acc[0][u]);
-                        acc[1][u] = mfma16(a1, bb, acc[1][u]);
+                    for (int u = 0; u < 5; ++u) {
+                        const int kt = wave + 4 * u;
+                        if (kt < KT) {
+                            // k = kt * 16 + pr: tap d = k / C (the same for the whole k tile), channel ci = k % C;
+                            // pixel p's input column is p + dxx - 1, staged at p + dxx + 7
+                            const int d = kt >> 1, ci = (kt & 1) * 16 + pr, dyy = d / 3, dxx = d % 3;
+                            const uint4 bb = ld_px8(XT + ci * XTP + (r + dyy) * XTW, p0 + dxx + 7, (dxx + 7) & 7);
+                            acc[0][u] = mfma32(a0, bb, acc[0][u]);
+                            acc[1][u] = mfma32(a1, bb, acc[1][u]);
+                        }
                     }
                 }
-            }
+        }
+        __syncthreads();  // every wavefront is done with its group's item
+        if (itn < it_end) store_item();
         __syncthreads();
     }
+    // the second group's sums into the first's (through the free staging space), then one atomic per sum
+    float* red = reinterpret_cast<float*>(&XTs[0][0]);  // 2 x 5 x 4 floats per lane of a group: 40 KB
+    if (grp == 1) {
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+        for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int u = 0; u < 5; ++u) {
-            const int kt = wave + 4 * u;
-            if (kt < KT)
+            for (int u = 0; u < 5; ++u)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) atomicAdd(&dw[size_t(16 * t + kq * 4 + j) * KK + kt * 16 + pr], acc[t][u][j]);
-        }
-    float* red = reinterpret_cast<float*>(DT);  // free after the last chunk's barrier
-    red[tid] = bsum;
+                for (int j = 0; j < 4; ++j) red[((t * 5 + u) * 4 + j) * 256 + tid] = acc[t][u][j];
+    }
+    float* bred = reinterpret_cast<float*>(&DTs[0][0]);
+    bred[threadIdx.x] = bsum;
     __syncthreads();
-    if (tid < C) {
+    if (grp == 0) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int u = 0; u < 5; ++u) {
+                const int kt = wave + 4 * u;
+                if (kt < KT)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        atomicAdd(&dw[size_t(16 * t + kq * 4 + j) * KK + kt * 16 + pr],
+                                  acc[t][u][j] + red[((t * 5 + u) * 4 + j) * 256 + tid]);
+            }
+    }
+    if (threadIdx.x < C) {
         float t = 0.f;
 #pragma unroll
-        for (int k = 0; k < 256 / C; ++k) t += red[tid + C * k];
-        atomicAdd(&db[tid], t);
+        for (int k = 0; k < WG_NT / C; ++k) t += bred[threadIdx.x + C * k];
+        atomicAdd(&db[threadIdx.x], t);
     }
 }
 
@@ -419,14 +486,14 @@ __global__ void k_weights_bf16(const float* w, uint16_t* wt, uint16_t* wf) {
     wf[size_t(ci) * KK + (8 - d) * C + co] = f2bf(w[i]);  // wf[ci][d' * 32 + co] = w[co][(8 - d') * 32 + ci]
 }
 
-__global__ void k_mfma_check(const uint16_t* a, const uint16_t* b, float* d) {
+__global__ void k_mfma_check(const uint16_t* a, const uint16_t* b, float* d) {  // a 16 x 32, b 32 x 16
     const int lane = threadIdx.x, pr = lane & 15, kq = lane >> 4;
-    v4s fa, fb;
-    for (int j = 0; j < 4; ++j) {
-        fa[j] = short(a[pr * 16 + kq * 4 + j]);  // A[pr][4 kq + j]
-        fb[j] = short(b[(kq * 4 + j) * 16 + pr]);  // B[4 kq + j][pr]
+    uint32_t fa[4], fb[4];
+    for (int j = 0; j < 4; ++j) {  // A[pr][8 kq + 2 j + h] and B[8 kq + 2 j + h][pr], h = 0, 1
+        fa[j] = uint32_t(a[pr * 32 + kq * 8 + 2 * j]) | uint32_t(a[pr * 32 + kq * 8 + 2 * j + 1]) << 16;
+        fb[j] = uint32_t(b[(kq * 8 + 2 * j) * 16 + pr]) | uint32_t(b[(kq * 8 + 2 * j + 1) * 16 + pr]) << 16;
     }
-    const v4f r = mfma16(fa, fb, v4f{0, 0, 0, 0});
+    const v4f r = mfma32(make_uint4(fa[0], fa[1], fa[2], fa[3]), make_uint4(fb[0], fb[1], fb[2], fb[3]), v4f{0, 0, 0, 0});
     for (int j = 0; j < 4; ++j) d[(kq * 4 + j) * 16 + pr] = r[j];  // D[4 kq + j][pr]
 }
 
@@ -453,6 +520,7 @@ struct DB {  // a device array grown on demand
 
 struct pcbprior {
     int cin = 0, width = 0, depth = 0, cout = 0;
+    int cus = 0;  // compute units of the device: the persistent kernels' grids
     std::vector<int> lin, lout;  // real channels of each layer
     hipStream_t s = nullptr;
     DB<float> w, g, mo, ve, bias, gb, mb, vb;  // per layer: C * KK weights, C biases (padded)
@@ -542,16 +610,28 @@ void collect(pcbprior* p) {  // after a stream sync
     p->pend.clear();
 }
 
+// persistent grids: at most as many blocks as the device holds at once (k_wgrad: each with an equal run of items)
+unsigned run_blocks(long work, long cap) {
+    const long per = (work + cap - 1) / cap;
+    return unsigned(std::max(1L, (work + per - 1) / per));
+}
+unsigned conv_blocks(const pcbprior* p, int n, int h, int w) {
+    return unsigned(std::max(1L, std::min(long(w / PX) * ((h + CR - 1) / CR) * n, long(CONV_BLOCKS_PER_CU) * p->cus)));
+}
+unsigned wgrad_blocks(const pcbprior* p, int n, int h, int w) {
+    return run_blocks(long(w / PX) * ((h + WR - 1) / WR) * n, long(WGRAD_BLOCKS_PER_CU) * p->cus);
+}
+
 // the layers over act[0] (already on the device)
 bool forward_dev(pcbprior* p, int n, int h, int w, const int32_t* valid = nullptr) {
-    const dim3 grid(w / PX, (h + CR - 1) / CR, n);
+    const dim3 grid(conv_blocks(p, n, h, w));
     const double np = double(n) * h * w;
     for (int L = 0; L < p->depth; ++L) {
         const bool last = L == p->depth - 1;
         Timed t(p, 0, 2.0 * np * p->lout[L] * 9 * p->lin[L]);
         hipLaunchKernelGGL(k_conv, grid, dim3(NT), 0, p->s, p->act[L].p, p->wt.p + size_t(L) * C * KK,
                            p->bias.p + L * C, last ? nullptr : p->act[L + 1].p, last ? p->logits.p : nullptr,
-                           static_cast<const uint16_t*>(nullptr), last ? 0 : 1, p->cout, h, w, valid);
+                           static_cast<const uint16_t*>(nullptr), last ? 0 : 1, p->cout, n, h, w, valid);
     }
     HIPOK(hipGetLastError());
     return true;
@@ -580,22 +660,22 @@ bool backward_dev(pcbprior* p, int n, int h, int w) {
     hipLaunchKernelGGL(k_count, dim3(g1(np)), dim3(256), 0, p->s, p->msk.p, np, p->cnt.p);
     hipLaunchKernelGGL(k_loss, dim3(unsigned(std::min<size_t>(g1(np), 2048))), dim3(256), 0, p->s, p->logits.p, p->tgt.p,
                        p->msk.p, p->dya.p, p->loss.p, np, p->cout, static_cast<const unsigned*>(p->cnt.p));
-    const dim3 grid(w / PX, (h + CR - 1) / CR, n);
+    const dim3 grid(conv_blocks(p, n, h, w)), wgrid(wgrad_blocks(p, n, h, w));
     uint16_t* dy = p->dya.p;
     uint16_t* dx = p->dyb.p;
     for (int L = p->depth - 1; L >= 0; --L) {
         const double fl = 2.0 * double(np) * p->lout[L] * 9 * p->lin[L];
         {
             Timed t(p, 1, fl);
-            hipLaunchKernelGGL(k_wgrad, dim3(w / PX, (h + WG_ROWS - 1) / WG_ROWS, n), dim3(256), 0, p->s, p->act[L].p,
-                               dy, p->g.p + size_t(L) * C * KK, p->gb.p + L * C, h, w);
+            hipLaunchKernelGGL(k_wgrad, wgrid, dim3(WG_NT), 0, p->s, p->act[L].p, dy, p->g.p + size_t(L) * C * KK,
+                               p->gb.p + L * C, n, h, w);
         }
         if (L > 0) {
             // the gradient at layer L's input: the flipped-weight conv of dy, zero where that input's ReLU was off
             Timed t(p, 0, fl);
             hipLaunchKernelGGL(k_conv, grid, dim3(NT), 0, p->s, dy, p->wf.p + size_t(L) * C * KK,
                                static_cast<const float*>(nullptr), dx, static_cast<float*>(nullptr), p->act[L].p, 0, 0,
-                               h, w, static_cast<const int32_t*>(nullptr));
+                               n, h, w, static_cast<const int32_t*>(nullptr));
             std::swap(dy, dx);
         }
     }
@@ -669,6 +749,13 @@ pcbprior* pcbprior_create(int in_ch, int width, int depth, int out_ch, uint64_t 
     }
     auto* p = new pcbprior;
     p->cin = in_ch; p->width = width; p->depth = depth; p->cout = out_ch;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        p->cus < 1) {
+        g_err = "no device to run on";
+        delete p;
+        return nullptr;
+    }
     for (int L = 0; L < depth; ++L) {
         p->lin.push_back(L == 0 ? in_ch : width);
         p->lout.push_back(L == depth - 1 ? out_ch : width);
@@ -867,13 +954,13 @@ void pcbprior_stats_read(pcbprior* p, pcbprior_stats* st, int reset) {
 }
 
 int pcbprior_mfma_check(const float* a, const float* b, float* d) {
-    uint16_t ha[256], hb[256];
-    for (int i = 0; i < 256; ++i) { ha[i] = f2bf(a[i]); hb[i] = f2bf(b[i]); }
+    uint16_t ha[512], hb[512];
+    for (int i = 0; i < 512; ++i) { ha[i] = f2bf(a[i]); hb[i] = f2bf(b[i]); }
     uint16_t *da = nullptr, *dbb = nullptr;
     float* dd = nullptr;
-    bool ok = hipMalloc(&da, 512) == hipSuccess && hipMalloc(&dbb, 512) == hipSuccess && hipMalloc(&dd, 1024) == hipSuccess &&
-              hipMemcpy(da, ha, 512, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(dbb, hb, 512, hipMemcpyHostToDevice) == hipSuccess;
+    bool ok = hipMalloc(&da, 1024) == hipSuccess && hipMalloc(&dbb, 1024) == hipSuccess && hipMalloc(&dd, 1024) == hipSuccess &&
+              hipMemcpy(da, ha, 1024, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(dbb, hb, 1024, hipMemcpyHostToDevice) == hipSuccess;
     if (ok) {
         hipLaunchKernelGGL(k_mfma_check, dim3(1), dim3(64), 0, 0, da, dbb, dd);
         ok = hipGetLastError() == hipSuccess && hipMemcpy(d, dd, 1024, hipMemcpyDeviceToHost) == hipSuccess;

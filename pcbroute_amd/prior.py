@@ -65,9 +65,9 @@ def library(path=LIB_PATH):
 
 
 def mfma_check(a, b):
-    """One 16x16x16 BF16 matrix-core product a @ b (inputs rounded to BF16, FP32 result)."""
+    """One 16x16x32 BF16 matrix-core product a (16x32) @ b (32x16) (inputs rounded to BF16, FP32 result)."""
     lib = library()
-    a, b = _f32(a, (16, 16)), _f32(b, (16, 16))
+    a, b = _f32(a, (16, 32)), _f32(b, (32, 16))
     d = np.zeros((16, 16), np.float32)
     if lib.pcbprior_mfma_check(_p(a), _p(b), _p(d)) != 0:
         raise BackendError(lib.pcbprior_last_error().decode())

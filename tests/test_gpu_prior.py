@@ -19,20 +19,26 @@ def prior():
 
 
 def test_mfma_lane_layout_exact_on_small_integers(prior):
+    """The 16x16x32 fragment layout the conv kernels assume: small integers multiply exactly in BF16 with FP32
+    sums, and one-hot operands pick out a single row or column, so any transposed, permuted or misplaced fragment
+    value changes the result."""
     rng = np.random.default_rng(0)
-    a = rng.integers(-8, 9, (16, 16)).astype(np.float32)
-    b = rng.integers(-8, 9, (16, 16)).astype(np.float32)
+    a = rng.integers(-8, 9, (16, 32)).astype(np.float32)
+    b = rng.integers(-8, 9, (32, 16)).astype(np.float32)
     assert np.array_equal(prior.mfma_check(a, b), a @ b)
-    e = np.zeros((16, 16), np.float32)
-    e[3, 11] = 1.0  # a single entry picks out one row of b: catches any transposed or permuted fragment
-    assert np.array_equal(prior.mfma_check(e, b), e @ b)
-    assert np.array_equal(prior.mfma_check(a, e), a @ e)
+    for (i, k) in ((3, 11), (15, 31), (0, 24), (9, 7)):
+        e = np.zeros((16, 32), np.float32)
+        e[i, k] = 1.0  # picks row k of b into row i
+        assert np.array_equal(prior.mfma_check(e, b), e @ b)
+        f = np.zeros((32, 16), np.float32)
+        f[k, i] = 1.0  # picks column k of a into column i
+        assert np.array_equal(prior.mfma_check(a, f), a @ f)
 
 
 def test_mfma_rounds_inputs_to_bf16(prior):
     rng = np.random.default_rng(1)
-    a = rng.standard_normal((16, 16)).astype(np.float32)
-    b = rng.standard_normal((16, 16)).astype(np.float32)
+    a = rng.standard_normal((16, 32)).astype(np.float32)
+    b = rng.standard_normal((32, 16)).astype(np.float32)
     want = R.bf16_round(a).astype(np.float64) @ R.bf16_round(b).astype(np.float64)
     np.testing.assert_allclose(prior.mfma_check(a, b), want, rtol=0, atol=1e-5)
 
