@@ -59,6 +59,13 @@ __host__ __device__ inline float bf2f(uint16_t h) {
     return f;
 }
 
+// 16 bytes at base + off if ok, else zeros: the load is always issued (at base when not ok, which must be valid), so
+// a staging loop is straight-line code with all its loads in flight together
+__device__ inline uint4 load16_or_0(const uint16_t* base, size_t off, bool ok) {
+    const uint4 v = *reinterpret_cast<const uint4*>(base + (ok ? off : 0));
+    return ok ? v : make_uint4(0, 0, 0, 0);
+}
+
 // D += A B on the matrix cores, the fragments as 16-byte vectors of 8 BF16 values
 __device__ inline v4f mfma32(uint4 a, uint4 b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
@@ -223,8 +230,8 @@ __global__ __launch_bounds__(NT) void k_conv(const uint16_t* x, const uint16_t* 
 // (two, and a shift, where the 3x3 tap's column offset misaligns it). A work item is WR rows of 64 columns of one
 // image (WR + 2 input rows staged); items are numbered row chunk first, then column block, then image. A block is
 // persistent and holds two groups of 4 wavefronts with a staging area each; it takes a contiguous run of items
-// (mostly one column strip downwards, so the input rows an item shares with the one before are fresh in L2), the
-// groups taking alternate items, each with its next item's loads in flight during the current item's products.
+// (mostly one column strip downwards), each group one half of it in order (so the input rows an item shares with the
+// one before are fresh in L2), with its next item's loads in flight during the current item's products.
 // Wavefront w of a group accumulates the k tiles w, w + 4, ... against both co tiles in registers over all its
 // items; at the end the second group's sums are added to the first's in LDS and the first adds them into dW with
 // FP32 atomics, once per block.
@@ -254,11 +261,38 @@ __device__ inline void put4(uint16_t* dst, int pitch, const uint4 (&v)[4]) {
         *reinterpret_cast<uint2*>(dst + k * pitch) = make_uint2(e[0] | (e[1] << 16), e[2] | (e[3] << 16));
     }
 }
+// One item's products for wavefront `wave` of a group: its k tiles are kt = wave + 4 u, whose tap is 2 u + wave / 2
+// and channel block (wave % 2) * 16, so with WP = wave / 2 known at compile time every tap's row and column offset,
+// and so every fragment's shift, is a constant, and the loop is straight-line code. All WR rows are taken: rows past
+// the image were staged as zeros and add nothing.
+template <int WP>
+__device__ inline void wgrad_item(const uint16_t* DT, const uint16_t* XT, int wave, int pr, int kq, v4f (&acc)[2][5]) {
+    constexpr int NU = WP == 0 ? 5 : 4;  // k tiles of waves 0, 1: up to 16, 17; of waves 2, 3: up to 14, 15
+    const int ci = (wave & 1) * 16 + pr;
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+        for (int ps = 0; ps < PX / 32; ++ps) {
+            // A = dy^T (row co, column pixel), B = x_im2col (row pixel, column k); K = 32 pixels of row r
+            const int p0 = ps * 32 + kq * 8;
+            const uint4 a0 = *reinterpret_cast<const uint4*>(DT + pr * DTP + r * PX + p0);
+            const uint4 a1 = *reinterpret_cast<const uint4*>(DT + (16 + pr) * DTP + r * PX + p0);
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                // pixel p's input column is p + dxx - 1, staged at p + dxx + 7
+                const int d = 2 * u + WP, dyy = d / 3, dxx = d % 3;
+                const uint4 bb = ld_px8(XT + ci * XTP + (r + dyy) * XTW, p0 + dxx + 7, (dxx + 7) & 7);
+                acc[0][u] = mfma32(a0, bb, acc[0][u]);
+                acc[1][u] = mfma32(a1, bb, acc[1][u]);
+            }
+        }
+}
 __global__ __launch_bounds__(WG_NT) void k_wgrad(const uint16_t* x, const uint16_t* dy, float* dw, float* db, int N,
                                                   int H, int W) {
     __shared__ __align__(16) uint16_t DTs[2][C * DTP];
     __shared__ __align__(16) uint16_t XTs[2][C * XTP];
-    const int grp = threadIdx.x >> 8, tid = threadIdx.x & 255, lane = tid & 63, wave = tid >> 6;
+    const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), tid = threadIdx.x & 255, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     uint16_t* DT = DTs[grp];
     uint16_t* XT = XTs[grp];
     const int pr = lane & 15, kq = lane >> 4;
@@ -277,25 +311,22 @@ __global__ __launch_bounds__(WG_NT) void k_wgrad(const uint16_t* x, const uint16
 #pragma unroll
         for (int it = 0; it < DIT; ++it) {
             const int e = tid + it * 256;
-            const int r = e / ((PX / 4) * (C / 8)), rem = e % ((PX / 4) * (C / 8)), g = rem / (C / 8), c8 = rem % (C / 8);
+            const int r = e / ((PX / 4) * (C / 8)), rem = e % ((PX / 4) * (C / 8)), c8 = rem / (PX / 4), g = rem % (PX / 4);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                vd[it][j] = (e < NDT && c0 + r < H)
-                                ? *reinterpret_cast<const uint4*>(dy + (img + size_t(c0 + r) * W + xb + 4 * g + j) * C + c8 * 8)
-                                : make_uint4(0, 0, 0, 0);
+                vd[it][j] = load16_or_0(dy, (img + size_t(c0 + r) * W + xb + 4 * g + j) * C + c8 * 8, e < NDT && c0 + r < H);
         }
 #pragma unroll
         for (int it = 0; it < XIT; ++it) {
             const int e = tid + it * 256;
-            const int r = e / (((PX + 8) / 4) * (C / 8)), rem = e % (((PX + 8) / 4) * (C / 8)), g = rem / (C / 8),
-                      c8 = rem % (C / 8);
+            const int r = e / (((PX + 8) / 4) * (C / 8)), rem = e % (((PX + 8) / 4) * (C / 8)), c8 = rem / ((PX + 8) / 4),
+                      g = rem % ((PX + 8) / 4);
             const int py = c0 - 1 + r;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int px = xb - 4 + 4 * g + j;
-                vx[it][j] = (e < NXT && py >= 0 && py < H && px >= 0 && px < W)
-                                ? *reinterpret_cast<const uint4*>(x + (img + size_t(py) * W + px) * C + c8 * 8)
-                                : make_uint4(0, 0, 0, 0);
+                vx[it][j] = load16_or_0(x, (img + size_t(py) * W + px) * C + c8 * 8,
+                                        e < NXT && py >= 0 && py < H && px >= 0 && px < W);
             }
         }
     };
@@ -303,14 +334,14 @@ __global__ __launch_bounds__(WG_NT) void k_wgrad(const uint16_t* x, const uint16
 #pragma unroll
         for (int it = 0; it < DIT; ++it) {
             const int e = tid + it * 256;
-            const int r = e / ((PX / 4) * (C / 8)), rem = e % ((PX / 4) * (C / 8)), g = rem / (C / 8), c8 = rem % (C / 8);
+            const int r = e / ((PX / 4) * (C / 8)), rem = e % ((PX / 4) * (C / 8)), c8 = rem / (PX / 4), g = rem % (PX / 4);
             if (e < NDT) put4(DT + (c8 * 8) * DTP + r * PX + 4 * g, DTP, vd[it]);
         }
 #pragma unroll
         for (int it = 0; it < XIT; ++it) {
             const int e = tid + it * 256;
-            const int r = e / (((PX + 8) / 4) * (C / 8)), rem = e % (((PX + 8) / 4) * (C / 8)), g = rem / (C / 8),
-                      c8 = rem % (C / 8);
+            const int r = e / (((PX + 8) / 4) * (C / 8)), rem = e % (((PX + 8) / 4) * (C / 8)), c8 = rem / ((PX + 8) / 4),
+                      g = rem % ((PX + 8) / 4);
             if (e < NXT) put4(XT + (c8 * 8) * XTP + r * XTW + 4 + 4 * g, XTP, vx[it]);  // column xb + c at c + 8
         }
     };
@@ -320,17 +351,19 @@ __global__ __launch_bounds__(WG_NT) void k_wgrad(const uint16_t* x, const uint16
 #pragma unroll
         for (int u = 0; u < 5; ++u) acc[t][u] = v4f{0, 0, 0, 0};
     float bsum = 0.f;
-    // group grp takes items it_beg + grp, it_beg + grp + 2, ...; both groups run the same number of rounds (barriers)
-    const int rounds = (max(0, it_end - it_beg) + 1) / 2;
-    if (it_beg + grp < it_end) {
-        load_item(it_beg + grp);
+    // group grp takes the grp-th half of the block's run, in order (consecutive row chunks: the input rows an item
+    // shares with the one before are fresh in L2); both groups run the same number of rounds (barriers)
+    const int rounds = (max(0, it_end - it_beg) + 1) / 2, g_beg = it_beg + grp * rounds;
+    const int g_end = min(it_end, g_beg + rounds);
+    if (g_beg < g_end) {
+        load_item(g_beg);
         store_item();
     }
     __syncthreads();
     for (int k = 0; k < rounds; ++k) {
-        const int it0 = it_beg + grp + 2 * k, itn = it0 + 2;
-        if (itn < it_end) load_item(itn);  // in flight during this item's products
-        if (it0 < it_end) {
+        const int it0 = g_beg + k, itn = it0 + 1;
+        if (itn < g_end) load_item(itn);  // in flight during this item's products
+        if (it0 < g_end) {
             const int rows = min(WR, H - (it0 % chunks) * WR);
             {  // the bias gradient: thread t sums 32 pixels of channel t % 32 (the partial sums meet at the end)
                 const int ch = tid & (C - 1), q0 = (tid / C) * 32;
@@ -345,29 +378,13 @@ __global__ __launch_bounds__(WG_NT) void k_wgrad(const uint16_t* x, const uint16
                         }
                     }
             }
-            for (int r = 0; r < rows; ++r)
-#pragma unroll
-                for (int ps = 0; ps < PX / 32; ++ps) {
-                    // A = dy^T (row co, column pixel), B = x_im2col (row pixel, column k); K = 32 pixels of row r
-                    const int p0 = ps * 32 + kq * 8;
-                    const uint4 a0 = *reinterpret_cast<const uint4*>(DT + pr * DTP + r * PX + p0);
-                    const uint4 a1 = *reinterpret_cast<const uint4*>(DT + (16 + pr) * DTP + r * PX + p0);
-#pragma unroll
-                    for (int u = 0; u < 5; ++u) {
-                        const int kt = wave + 4 * u;
-                        if (kt < KT) {
-                            // k = kt * 16 + pr: tap d = k / C (the same for the whole k tile), channel ci = k % C;
-                            // pixel p's input column is p + dxx - 1, staged at p + dxx + 7
-                            const int d = kt >> 1, ci = (kt & 1) * 16 + pr, dyy = d / 3, dxx = d % 3;
-                            const uint4 bb = ld_px8(XT + ci * XTP + (r + dyy) * XTW, p0 + dxx + 7, (dxx + 7) & 7);
-                            acc[0][u] = mfma32(a0, bb, acc[0][u]);
-                            acc[1][u] = mfma32(a1, bb, acc[1][u]);
-                        }
-                    }
-                }
+            if (wave < 2)
+                wgrad_item<0>(DT, XT, wave, pr, kq, acc);
+            else
+                wgrad_item<1>(DT, XT, wave, pr, kq, acc);
         }
         __syncthreads();  // every wavefront is done with its group's item
-        if (itn < it_end) store_item();
+        if (itn < g_end) store_item();
         __syncthreads();
     }
     // the second group's sums into the first's (through the free staging space), then one atomic per sum
