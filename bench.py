@@ -289,11 +289,18 @@ def prior_workload(b, a):
     px = float(batch) * h * w
     real = 250 * 300 / (h * w)
     step_flop = st["conv_flop"] / a.steps + st["wgrad_flop"] / a.steps
-    kern = {"k_conv": (st["conv_flop"], st["conv_ms"], st["conv_launches"]),
-            "k_wgrad": (st["wgrad_flop"], st["wgrad_ms"], st["wgrad_launches"])}
+    # algorithmic HBM bytes (activations are 32 BF16 slots, 64 B per pixel): a forward conv reads its input and writes
+    # its output (the last one writes cout FP32 logits); an input-gradient conv reads dy and the mask activation and
+    # writes dx; a weight-gradient launch reads the layer's input and dy
+    depth = 6
+    conv_b = a.steps * px * ((depth - 1) * 128 + 64 + 4 * cout + (depth - 1) * 192)
+    wgrad_b = a.steps * px * depth * 128
+    kern = {"k_conv": (st["conv_flop"], st["conv_ms"], st["conv_launches"], conv_b),
+            "k_wgrad": (st["wgrad_flop"], st["wgrad_ms"], st["wgrad_launches"], wgrad_b)}
     dom = max(kern, key=lambda k: kern[k][1])
-    fl, ms, nlaunch = kern[dom]
+    fl, ms, nlaunch, nbytes = kern[dom]
     achieved = fl / (ms / 1e3) / 1e12
+    gbs = nbytes / (ms / 1e3) / 1e9
     line = {
         "metric": "prior training boards/s", "value": round(a.steps * batch / wall, 3), "unit": "boards/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 3),
@@ -305,11 +312,19 @@ def prior_workload(b, a):
         "inference_boards_per_s": round(a.steps * batch / wall_inf, 3),
         "train_tflops": round(step_flop / (wall / a.steps) / 1e12, 2),
         "useful_fraction_of_pixels": round(real, 3),
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": PRIOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PRIOR_PEAK_TFLOPS, 5), "traffic": None, "kernel": dom,
+        # both kernels sit below the ridge point (2.5 PFLOP/s / 8 TB/s = 312 FLOP/B): ~100-150 FLOP per algorithmic
+        # byte, so HBM bounds them; the matrix-core fraction is given beside it
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(gbs / 8000.0, 4), "traffic": None, "kernel": dom,
                      "launches": nlaunch, "avg_launch_us": round(ms * 1e3 / max(1, nlaunch), 2),
-                     "flop_per_launch": round(fl / max(1, nlaunch))},
-        "kernels": {k: {"tflops": round(v[0] / (v[1] / 1e3) / 1e12, 2), "ms": round(v[1], 3), "launches": v[2]}
+                     "bytes_per_launch": round(nbytes / max(1, nlaunch)), "flop_per_launch": round(fl / max(1, nlaunch)),
+                     "flop_per_byte": round(fl / nbytes, 1),
+                     "mfma_tflops": round(achieved, 2), "mfma_peak": PRIOR_PEAK_TFLOPS,
+                     "mfma_frac": round(achieved / PRIOR_PEAK_TFLOPS, 5)},
+        "kernels": {k: {"tflops": round(v[0] / (v[1] / 1e3) / 1e12, 2), "mfma_frac": round(v[0] / (v[1] / 1e3) / 1e12
+                                                                                       / PRIOR_PEAK_TFLOPS, 4),
+                        "gb_s": round(v[3] / (v[1] / 1e3) / 1e9, 1), "hbm_frac": round(v[3] / (v[1] / 1e3) / 8e12, 4),
+                        "ms": round(v[1], 3), "launches": v[2]}
                     for k, v in kern.items()},
     }
     if not a.no_cpu:

@@ -422,13 +422,15 @@ __global__ __launch_bounds__(WG_NT) void k_wgrad(const uint16_t* x, const uint16
 }
 
 // binary cross-entropy of the logits over the masked pixels; dz = (sigmoid(z) - t) m / count into the BF16 slots
-// (0 in the padded ones). A thread takes pixels in a grid-stride loop; one atomic per block adds its partial loss.
+// (0 in the padded ones). A thread per pixel; each block adds its partial loss into one of LOSS_SLOTS sums (atomics
+// on one address serialize), which the host adds up.
+constexpr int LOSS_SLOTS = 64;
 __global__ __launch_bounds__(256) void k_loss(const float* z, const float* t, const float* m, uint16_t* dz, float* loss,
                                               size_t npix, int cout, const unsigned* cnt) {
     __shared__ float part[4];
     const float inv = *cnt ? 1.f / (float(*cnt) * float(cout)) : 0.f;
     float l = 0.f;
-    for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < npix; p += size_t(gridDim.x) * blockDim.x) {
+    if (const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < npix) {
         const bool on = m[p] != 0.f;
         uint32_t w[C / 2];
         for (int c = 0; c < C; c += 2) {
@@ -447,14 +449,16 @@ __global__ __launch_bounds__(256) void k_loss(const float* z, const float* t, co
     for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(loss, part[0] + part[1] + part[2] + part[3]);
+    if (threadIdx.x == 0) atomicAdd(&loss[blockIdx.x % LOSS_SLOTS], part[0] + part[1] + part[2] + part[3]);
 }
 
-// masked pixels of a batch (the loss's mean runs over them), one atomic per block
+// masked pixels of a batch (the loss's mean runs over them): a grid-stride loop, one atomic per block (a few
+// hundred blocks: atomics on one address serialize)
 __global__ __launch_bounds__(256) void k_count(const float* m, size_t npix, unsigned* cnt) {
     __shared__ unsigned part[4];
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    unsigned c = (i < npix && m[i] != 0.f) ? 1u : 0u;
+    unsigned c = 0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < npix; i += size_t(gridDim.x) * blockDim.x)
+        c += m[i] != 0.f ? 1u : 0u;
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
     __syncthreads();
@@ -495,12 +499,24 @@ __global__ void k_adam(float* w, const float* g, float* mo, float* ve, size_t n,
     w[i] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
 }
 
+// the BF16 copies of every layer's weights (blockIdx.y: the layer)
 __global__ void k_weights_bf16(const float* w, uint16_t* wt, uint16_t* wf) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;  // over C * KK
     if (i >= C * KK) return;
+    const size_t L = size_t(blockIdx.y) * C * KK;
     const int co = i / KK, k = i % KK, d = k / C, ci = k % C;
-    wt[i] = f2bf(w[i]);
-    wf[size_t(ci) * KK + (8 - d) * C + co] = f2bf(w[i]);  // wf[ci][d' * 32 + co] = w[co][(8 - d') * 32 + ci]
+    const uint16_t v = f2bf(w[L + i]);
+    wt[L + i] = v;
+    wf[L + size_t(ci) * KK + (8 - d) * C + co] = v;  // wf[ci][d' * 32 + co] = w[co][(8 - d') * 32 + ci]
+}
+
+// a backward pass's accumulators to zero in one launch: the weight and bias gradients, the loss and the pixel count
+__global__ void k_zero_grads(float* g, size_t ng, float* gb, size_t ngb, float* loss, unsigned* cnt) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < ng) g[i] = 0.f;
+    if (i < ngb) gb[i] = 0.f;
+    if (i < LOSS_SLOTS) loss[i] = 0.f;
+    if (i == 0) *cnt = 0u;
 }
 
 __global__ void k_mfma_check(const uint16_t* a, const uint16_t* b, float* d) {  // a 16 x 32, b 32 x 16
@@ -544,7 +560,8 @@ struct pcbprior {
     DB<uint16_t> wt, wf;
     std::vector<DB<uint16_t>> act;  // depth activations: act[0] the input, act[L] the input of layer L
     DB<uint16_t> dya, dyb;
-    DB<float> logits, tgt, msk, xin, loss;
+    DB<float> logits, tgt, msk, xin, loss;  // loss: LOSS_SLOTS partial sums
+    float loss_h[LOSS_SLOTS] = {};          // their host copy
     DB<unsigned> cnt;
     DB<int32_t> valid;  // per image (rows, columns) of a padded batch
     int step = 0;
@@ -572,6 +589,13 @@ bool d2h(pcbprior* p, void* dst, const void* src, size_t bytes) {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, p->s) == hipSuccess || fail("hipMemcpyAsync D2H");
 }
 bool sync(pcbprior* p) { return hipStreamSynchronize(p->s) == hipSuccess || fail("hipStreamSynchronize"); }
+// the loss's partial sums to the host (ordered on the stream: read them after a sync), and their total
+bool loss_to_host(pcbprior* p) { return d2h(p, p->loss_h, p->loss.p, LOSS_SLOTS * 4); }
+float loss_sum(const pcbprior* p) {
+    float t = 0.f;
+    for (int k = 0; k < LOSS_SLOTS; ++k) t += p->loss_h[k];
+    return t;
+}
 
 bool ensure(pcbprior* p, int n, int h, int w) {
     if (n < 1 || h < 1 || w < 1 || w % PX) return fail("n, h >= 1 and w a positive multiple of 64");
@@ -581,13 +605,12 @@ bool ensure(pcbprior* p, int n, int h, int w) {
     for (auto& a : p->act)
         if (!a.need(np * C)) return false;
     return p->dya.need(np * C) && p->dyb.need(np * C) && p->logits.need(np * p->cout) && p->tgt.need(np * p->cout) &&
-           p->msk.need(np) && p->xin.need(np * p->cin) && p->loss.need(1);
+           p->msk.need(np) && p->xin.need(np * p->cin) && p->loss.need(LOSS_SLOTS);
 }
 
 bool refresh_bf16(pcbprior* p) {
-    for (int L = 0; L < p->depth; ++L)
-        hipLaunchKernelGGL(k_weights_bf16, dim3(g1(C * KK)), dim3(256), 0, p->s, p->w.p + size_t(L) * C * KK,
-                           p->wt.p + size_t(L) * C * KK, p->wf.p + size_t(L) * C * KK);
+    hipLaunchKernelGGL(k_weights_bf16, dim3(g1(C * KK), unsigned(p->depth)), dim3(256), 0, p->s, p->w.p, p->wt.p,
+                       p->wf.p);
     HIPOK(hipGetLastError());
     return true;
 }
@@ -670,12 +693,11 @@ bool forward(pcbprior* p, const float* x, int n, int h, int w, const int32_t* va
 bool backward_dev(pcbprior* p, int n, int h, int w) {
     const size_t np = p->npix;
     if (!p->cnt.need(1)) return false;
-    HIPOK(hipMemsetAsync(p->loss.p, 0, 4, p->s));
-    HIPOK(hipMemsetAsync(p->cnt.p, 0, 4, p->s));
-    HIPOK(hipMemsetAsync(p->g.p, 0, size_t(p->depth) * C * KK * 4, p->s));
-    HIPOK(hipMemsetAsync(p->gb.p, 0, size_t(p->depth) * C * 4, p->s));
-    hipLaunchKernelGGL(k_count, dim3(g1(np)), dim3(256), 0, p->s, p->msk.p, np, p->cnt.p);
-    hipLaunchKernelGGL(k_loss, dim3(unsigned(std::min<size_t>(g1(np), 2048))), dim3(256), 0, p->s, p->logits.p, p->tgt.p,
+    const size_t ng = size_t(p->depth) * C * KK, ngb = size_t(p->depth) * C;
+    hipLaunchKernelGGL(k_zero_grads, dim3(g1(ng)), dim3(256), 0, p->s, p->g.p, ng, p->gb.p, ngb, p->loss.p, p->cnt.p);
+    hipLaunchKernelGGL(k_count, dim3(unsigned(std::min<size_t>(g1(np), size_t(p->cus)))), dim3(256), 0, p->s, p->msk.p,
+                       np, p->cnt.p);  // one atomic per block
+    hipLaunchKernelGGL(k_loss, dim3(g1(np)), dim3(256), 0, p->s, p->logits.p, p->tgt.p,
                        p->msk.p, p->dya.p, p->loss.p, np, p->cout, static_cast<const unsigned*>(p->cnt.p));
     const dim3 grid(conv_blocks(p, n, h, w)), wgrid(wgrad_blocks(p, n, h, w));
     uint16_t* dy = p->dya.p;
@@ -706,8 +728,8 @@ bool backward(pcbprior* p, const float* target, const float* mask, int n, int h,
     HIPOK(hipMemcpyAsync(p->msk.p, mask, np * 4, hipMemcpyHostToDevice, p->s));
     if (!backward_dev(p, n, h, w)) return false;
     if (loss_out) {
-        HIPOK(hipMemcpyAsync(loss_out, p->loss.p, 4, hipMemcpyDeviceToHost, p->s));
-        HIPOK(hipStreamSynchronize(p->s));
+        if (!loss_to_host(p) || !sync(p)) return false;
+        *loss_out = loss_sum(p);
         collect(p);
     }
     return true;
@@ -915,8 +937,9 @@ int pcbprior_train_step(pcbprior* p, const float* x, const float* target, const 
     g_err.clear();
     if (!p || !x || !target || !mask) return -1;
     if (!forward(p, x, n, h, w) || !backward(p, target, mask, n, h, w, nullptr) || !adam(p, lr)) return -1;
-    if (loss && !d2h(p, loss, p->loss.p, 4)) return -1;
+    if (loss && !loss_to_host(p)) return -1;
     if (!sync(p)) return -1;
+    if (loss) *loss = loss_sum(p);
     collect(p);
     return 0;
 }
@@ -944,8 +967,9 @@ int pcbprior_train_resident(pcbprior* p, const int32_t* idx, int batch, float lr
     if (!gather(p, idx, batch) || !forward_dev(p, batch, p->ds_h, p->ds_w) || !backward_dev(p, batch, p->ds_h, p->ds_w) ||
         !adam(p, lr))
         return -1;
-    if (loss && !d2h(p, loss, p->loss.p, 4)) return -1;
+    if (loss && !loss_to_host(p)) return -1;
     if (!sync(p)) return -1;
+    if (loss) *loss = loss_sum(p);
     collect(p);
     return 0;
 }
