@@ -10,8 +10,8 @@ roofline: the sweep kernel k_pass, algorithmic bytes = 12 B per state per tile v
 field read and write, once per visit however many local sweeps it runs) / summed kernel time from HIP events on the
 launch stream. Beside it: the HBM bytes rocprofv3's FETCH_SIZE and WRITE_SIZE counters measured per launch of the
 same configuration (profiles/), their rate, and the figure that credits every local sweep with 12 B.
-Tile rows, local sweeps and the delta band: the library's defaults for this workload (16-row tiles, up to 4 local
-sweeps, automatic band), so the committed rocprof traffic profile matches the run; --tune auto runs the autotuner on
+Tile rows, local sweeps and the delta band: the library's defaults for this workload (16-row tiles, up to 2 local
+sweeps, automatic band of 20 straight steps), so the committed rocprof traffic profile matches the run; --tune auto runs the autotuner on
 a sample drawn with another seed first. The batched round workload tunes by default.
 
 --gpus N with N > 1 exits: the back end solves each search on a single device.
@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 ALGO_BYTES = 12        # per state-sweep (a state carried through the eight directional relaxations and the vias):
                        # int32 cost word read, int32 field read + write
-DEFAULT_DELTA = 0      # delta-stepping band over tiles: 0 = the library's automatic band (80 straight steps)
+DEFAULT_DELTA = 0      # delta-stepping band over tiles: 0 = the library's automatic band (20 straight steps)
 
 
 def log(*a):

@@ -196,11 +196,11 @@ void pcbgpu_round_end(pcbgpu_round* r);
 /* tile rows per workgroup (16, 32, -1: by layer count) and local sweeps per tile visit; 0 keeps the current value,
  * (-1, -1) restores the defaults. Returns 0 or -1. By default routing windows (rectangles of at most 512 tiles of
  * 64 x 16 cells, whose passes are latency-bound) run 32 rows and 2 local sweeps and larger rectangles rows by layer
- * count and 4 sweeps (pcbgpu_get_tile); a setting serves every rectangle. Results never depend on it. */
+ * count and 2 sweeps (pcbgpu_get_tile); a setting serves every rectangle. Results never depend on it. */
 int pcbgpu_set_tile(int32_t tile_rows, int32_t local_sweeps);
 void pcbgpu_get_tile(int32_t nl, int32_t* tile_rows, int32_t* local_sweeps);
 /* delta-stepping band over tiles: a pass visits the queued tiles whose key (the least cost change that queued them)
- * lies within delta of the least key; 0 (the default) picks 80 times the cheapest straight move weight, or no band
+ * lies within delta of the least key; 0 (the default) picks 20 times the cheapest straight move weight, or no band
  * for rectangles of at most 512 tiles; -1 visits every queued tile. Results do not depend on it. 0 or -1. */
 int pcbgpu_set_delta(int32_t delta);
 /* ---- autotuner (no counterpart in the reference). Tile rows per wavefront, local sweeps per tile visit and the
@@ -214,7 +214,7 @@ typedef struct pcbgpu_tune {
     double ms;             /* the sample round's time under this setting */
 } pcbgpu_tune;
 /* Times the sample round (n searches, qs[i] on grids[i], one layer count) under each candidate by coordinate
- * descent (rows, then local sweeps, then delta, best of reps runs each), checks that every candidate returns the same
+ * descent (rows, then two rounds of local sweeps and delta, best of reps runs each), checks that every candidate returns the same
  * best states and rectangles, installs the fastest for that layer count and copies it to best. Returns the number
  * of candidates timed, -1 on error (the previous setting stays). Call it while no other solve runs. */
 int pcbgpu_autotune(const pcbgpu_grid* const* grids, const pcbgpu_query* const* qs, int n, int reps, pcbgpu_tune* best);

@@ -109,7 +109,7 @@ def test_tile_setting_and_defaults():
             b.set_tile(16, -3)
     finally:
         b.set_tile(-1, -1)
-    assert b.get_tile(4) == default and default[1] == 4
+    assert b.get_tile(4) == default and default[1] == 2
 
 
 def test_out_at_reads_the_solved_rectangle_only():

@@ -272,12 +272,15 @@ def _naive_backtrace(p, cf, vok, vcf, d, best):
         q = first[1]
 
 
-@pytest.mark.parametrize("seed", range(8))
+# seeds of random_case(seed, 40, 32, 2 + seed % 2) whose targets are reachable (5002 and 5007 are not)
+FIRST_MIN_SEEDS = (5000, 5001, 5003, 5004, 5005, 5006, 5008, 5009)
+
+
+@pytest.mark.parametrize("seed", FIRST_MIN_SEEDS)
 def test_backtrace_first_minimum(oracle, seed):
-    p = random_case(5000 + seed, nx=40, ny=32, nl=2 + seed % 2, features=False)
+    p = random_case(seed, nx=40, ny=32, nl=2 + seed % 2, features=False)
     res = oracle.solve(p)
-    if res["best"] < 0:
-        pytest.skip("no target reachable")
+    assert res["best"] >= 0
     assert res["rect"] == (0, 0, p.window[2], p.window[3])
     cf, vok, tg, vcf = oracle.resolve(p)
     chain = oracle.backtrace(p, res["rect"], res["best"], res["d"])
