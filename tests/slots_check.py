@@ -4,6 +4,7 @@ import json
 import os
 import sys
 import threading
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
@@ -50,6 +51,7 @@ def main():
     holders = [threading.Thread(target=hold) for _ in range(out["slots"])]
     for t in holders:
         t.start()
+    time.sleep(0.05)  # the holders are inside their first searches
     busy = ok = 0
     q = sp.query()
     for _ in range(400):
@@ -60,6 +62,9 @@ def main():
             busy += 1
         elif rc >= 0:
             ok += 1
+            # a poll that found a slot free took it; give the holder waiting for it time to take it back, or the
+            # next poll wins the race again and the holders never run
+            time.sleep(0.005)
         if busy >= 3:
             break
     waited = b.solve(sp, wait=True)
