@@ -88,7 +88,8 @@ typedef struct pcbgpu_out {
     int64_t best;
     int32_t* d;
     size_t cap;
-    int32_t sweeps;       /* passes of the last rectangle's solve */
+    int32_t sweeps;       /* passes of the last rectangle's solve (the field is relaxed in place: may differ by a few
+                             between two solves of one query; the field, rectangle and best state do not) */
     int32_t tries;        /* rectangles solved */
     double ms_upload, ms_solve, ms_total;
     uint64_t tile_visits; /* tiles the sweep kernels processed (device work, all rectangles) */
