@@ -8,7 +8,8 @@ the same searches one after another (one solve at a time) are timed beside it. P
 value: grid states solved per second (64 x 12.8M states per step / wall time of the round mode).
 roofline: the sweep kernel k_pass, algorithmic bytes = 12 B per state per tile visit (int32 cost word read, int32
 field read and write, once per visit however many local sweeps it runs) / summed kernel time from HIP events on the
-launch stream. Beside it: the HBM bytes rocprofv3's FETCH_SIZE and WRITE_SIZE counters measured per launch of the
+launch stream, taken in as many steps run after the timed ones (the events launch the passes one by one; the timed
+steps replay them from captured hipGraph chunks). Beside it: the HBM bytes rocprofv3's FETCH_SIZE and WRITE_SIZE counters measured per launch of the
 same configuration (profiles/), their rate, and the figure that credits every local sweep with 12 B.
 Tile rows, local sweeps and the delta band: the library's defaults for this workload (16-row tiles, up to 2 local
 sweeps, automatic band of 20 straight steps), so the committed rocprof traffic profile matches the run; --tune auto runs the autotuner on
@@ -93,14 +94,20 @@ def run(b, g, probs, mode, steps, warmup):
     for w in range(warmup):
         step()
         log(f"{mode}: warmup {w + 1}/{warmup} done")
-    b.stats(enable=True, reset=True)
+    # the timed steps run the production path (a round replays its passes from captured hipGraph chunks); the kernel
+    # accounting (HIP events around every pass, which launches passes one by one) runs in as many steps after them
+    b.stats(enable=False, reset=True)
     t0 = time.perf_counter()
     sweeps = []
     for k in range(steps):
         sweeps += step()
         log(f"{mode}: step {k + 1}/{steps} done")
     wall = time.perf_counter() - t0
+    b.stats(enable=True, reset=True)
+    for k in range(steps):
+        step()
     st = b.stats(enable=False, reset=True)
+    log(f"{mode}: {steps} accounted steps done")
     return wall, st, sweeps
 
 

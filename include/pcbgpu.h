@@ -180,7 +180,8 @@ int pcbgpu_solve_grid_noalloc(const pcbgpu_grid* g, const pcbgpu_query* q, int32
  * pcbgpu_round_launch returns. All searches of a round share one layer count. */
 typedef struct pcbgpu_round pcbgpu_round;
 pcbgpu_round* pcbgpu_round_begin(void);
-/* a ticket >= 0, or PCBGPU_ERR_FAIL */
+/* a ticket >= 0, or PCBGPU_ERR_FAIL (a query without a start, or one that does not fit the grid). The target bitmaps
+ * are scanned at launch, on host threads: a search without a target fails at its pcbgpu_round_result. */
 int pcbgpu_round_submit(pcbgpu_round* r, const pcbgpu_grid* g, const pcbgpu_query* q);
 /* the same for one given rectangle of the window, without widening (pcbgpu_solve_rect_grid's result) */
 int pcbgpu_round_submit_rect(pcbgpu_round* r, const pcbgpu_grid* g, const pcbgpu_query* q, int32_t rx0, int32_t ry0,

@@ -292,6 +292,39 @@ def test_round_equals_one_at_a_time(backend, oracle, nl):
             g.close()
 
 
+def test_round_search_without_target_fails_alone(backend, oracle):
+    """A round scans its target bitmaps at launch: a search without a target fails at its own result (with the
+    reason) while the whole-grid searches beside it (bitmaps large enough to be scanned for their nonzero words, on
+    several host threads) still solve exactly."""
+    import ctypes as C
+    from pcbroute_amd.abi import Out
+    board = synth.kernel_board(5, nx=1200, ny=1000, nl=4)
+    good = synth.pair_problems(board, 9, 4)
+    p0 = good[0]
+    none = Problem(p0.cst, p0.st, p0.vcst, p0.vst, starts=p0.starts, targets=[], window=p0.window)
+    g = backend.grid(p0)
+    lib = backend.lib
+    try:
+        r = lib.pcbgpu_round_begin()
+        qs = [good[0].query(), none.query()] + [p.query() for p in good[1:]]
+        tickets = [lib.pcbgpu_round_submit(r, g.h, C.byref(q)) for q in qs]
+        assert tickets == [0, 1, 2, 3, 4]
+        assert lib.pcbgpu_round_launch(r) == 0
+        rcs, errs = [], []
+        for t in tickets:
+            o = Out()
+            rcs.append(lib.pcbgpu_round_result(r, t, C.byref(o)))
+            errs.append(backend.error())
+            lib.pcbgpu_out_release(C.byref(o))
+        lib.pcbgpu_round_end(r)
+        assert rcs[1] < 0 and all(rc > 0 for k, rc in enumerate(rcs) if k != 1)
+        assert "no start or no target" in errs[1]
+        for p, got in zip(good, backend.round([(g, q) for q in good], field=True)):
+            _same(got, oracle.solve(p))
+    finally:
+        g.close()
+
+
 def test_c3_round_against_the_oracle(backend, oracle):
     """The batched speculative round at the C3 board's shape, with fewer and smaller searches: a 2400 x 1800 x 6 board
     seen by two net classes (clearances of 1 and 2 cells), 32 searches of 2-8 pins whose boxes are log-uniform in
