@@ -111,6 +111,9 @@ typedef struct pcbr_options {
                                  next round would pass half of it, legalizing follows, the rescue pass gets what is
                                  left */
     double rescue_s;          /* the rescue pass's own cap (0: half the time the route took so far, at least 10 s) */
+    int32_t ripups;           /* while legalizing, times a net left open may push its way through others' copper, the
+                                 nets it crosses then routing around it (src/router.h:42, src/router.cpp:2296-2340;
+                                 3, the default; 0: none) */
 } pcbr_options;
 void pcbr_default_options(pcbr_options* o);
 
@@ -130,6 +133,7 @@ typedef struct pcbr_result {
     int32_t first_conflict_nets, first_conflict_cells;  /* left in conflict by the first negotiation round */
     int32_t rescue_tried, rescue_kept, opens_before_rescue;
     int32_t limited;          /* 1: the time limit cut negotiation short, 2: also the rescue pass */
+    int32_t ripups;           /* pushes made while legalizing */
     uint64_t solved_states;   /* states the searches covered: each device search's solved rectangle, each CPU
                                  search's window (the work a search costs, whichever side ran it) */
 } pcbr_result;

@@ -37,7 +37,7 @@ class Options(C.Structure):
                 ("negotiate", C.c_int32), ("rescue", C.c_int32), ("smooth", C.c_int32),
                 ("ml_rounds", C.c_int32), ("gpu_pops", C.c_int64), ("max_pops", C.c_int64),
                 ("probe_pops", C.c_int64), ("gpu_wait", C.c_int32), ("time_limit_s", C.c_double),
-                ("rescue_s", C.c_double)]
+                ("rescue_s", C.c_double), ("ripups", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -52,7 +52,8 @@ class Result(C.Structure):
                                          "probe_opens", "budget_opens")] + \
                [("cpu_pops", C.c_uint64), ("cpu_s", C.c_double), ("first_conflict_nets", C.c_int32),
                 ("first_conflict_cells", C.c_int32), ("rescue_tried", C.c_int32), ("rescue_kept", C.c_int32),
-                ("opens_before_rescue", C.c_int32), ("limited", C.c_int32), ("solved_states", C.c_uint64)]
+                ("opens_before_rescue", C.c_int32), ("limited", C.c_int32), ("ripups", C.c_int32),
+                ("solved_states", C.c_uint64)]
 
 
 SEARCH_CB = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(Query), C.POINTER(Out))

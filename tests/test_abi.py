@@ -44,12 +44,13 @@ def test_library_exports_every_declared_function():
 
 
 def test_struct_layout_matches_c():
-    fields = {
-        "pcbgpu_query": [f[0] for f in Query._fields_],
-        "pcbgpu_out": [f[0] for f in Out._fields_],
-        "pcbgpu_entry": [f[0] for f in Entry._fields_],
-    }
-    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void) {']
+    from pcbroute_amd.router import Options, Result
+    classes = {"pcbgpu_query": Query, "pcbgpu_out": Out, "pcbgpu_entry": Entry, "pcbr_options": Options,
+               "pcbr_result": Result}
+    fields = {s: [f[0] for f in cls._fields_] for s, cls in classes.items()}
+    router_h = os.path.join(os.path.dirname(HEADER), "pcbroute.h")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', f'#include "{router_h}"',
+             'int main(void) {']
     for s, fs in fields.items():
         lines.append(f'printf("{s} size %zu\\n", sizeof({s}));')
         for f in fs:
@@ -61,7 +62,7 @@ def test_struct_layout_matches_c():
         subprocess.run(["gcc", "-o", exe, src], check=True)
         got = dict(line.rsplit(" ", 1) for line in subprocess.run([exe], capture_output=True, text=True,
                                                                    check=True).stdout.splitlines())
-    for s, cls in (("pcbgpu_query", Query), ("pcbgpu_out", Out), ("pcbgpu_entry", Entry)):
+    for s, cls in classes.items():
         assert int(got[f"{s} size"]) == C.sizeof(cls), s
         for f in fields[s]:
             assert int(got[f"{s}.{f}"]) == getattr(cls, f).offset, (s, f)
