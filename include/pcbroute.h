@@ -108,8 +108,8 @@ typedef struct pcbr_options {
     int32_t gpu_wait;         /* wait for a free device slot (1, the default) or, when all are busy, go on on the
                                  CPU for 200,000 pops and try again (0, --gpu-nowait) */
     double time_limit_s;      /* a route's time limit (src/router.cpp:2247-2256; 0: none): negotiation stops when the
-                                 next round would pass half of it, legalizing follows, the rescue pass gets what is
-                                 left */
+                                 next round would pass half of it, legalizing follows with rip-ups until 80 % of it,
+                                 the rescue pass gets what is left */
     double rescue_s;          /* the rescue pass's own cap (0: half the time the route took so far, at least 10 s) */
     int32_t ripups;           /* while legalizing, times a net left open may push its way through others' copper, the
                                  nets it crosses then routing around it (src/router.h:42, src/router.cpp:2296-2340;
