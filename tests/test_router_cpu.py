@@ -254,3 +254,25 @@ def test_probe_leaves_walled_in_pins_as_the_full_search_does(lib):
     plain = R.Router(b, R.default_options(L, gpu_pops=0), lib_path=lib).route()
     probed = R.Router(b, R.default_options(L, gpu_pops=0, probe_pops=5000), lib_path=lib).route()
     assert probed["route_hash"] == plain["route_hash"] and probed["opens"] == plain["opens"]
+
+
+@pytest.mark.parametrize("seed", [2, 3])
+def test_ripups_while_legalizing(lib, seed):
+    """A one-layer board negotiated for two rounds leaves nets in conflict; legalizing them around the others leaves
+    connections open, and the rip-ups (an open net pushing through the others, the nets it crossed legalizing around
+    it) close them. Either way the board ends legal, as the router's own check and its result agree."""
+    def route(ripups):
+        s = R.spec("tiny", seed=seed)
+        s.layers = 1
+        b = R.Board(s, lib_path=lib)
+        r = R.Router(b, R.default_options(R.library(lib), max_iters=2, ripups=ripups, rescue=0), lib_path=lib)
+        res = r.route()
+        ev = r.eval()
+        r.close()
+        b.close()
+        return res, ev
+    plain, ev0 = route(0)
+    ripped, ev3 = route(3)
+    assert ev0 == (plain["opens"], 0) and ev3 == (ripped["opens"], 0)
+    assert plain["ripups"] == 0 and plain["opens"] > 0
+    assert ripped["ripups"] > 0 and ripped["opens"] < plain["opens"]
